@@ -1,0 +1,1157 @@
+// Device engine of the MI355X NUFHT: builds a "program" (flat work lists for
+// every stage over all boxes of one or many transforms), runs it on a HIP
+// stream, and implements the device-side entry points of nufht_amd.h plus
+// the host-pointer nufht_batch of the drop-in ABI.
+//
+// Stage order per program (one launch per stage, all transforms at once):
+//   zero gs -> local projection -> local evaluation -> direct tiles
+//   -> asymptotic spread -> rocFFT (one plan per grid-length class)
+//   -> asymptotic interpolation
+// which covers the reference's execute_boxes_batch (src/fast_hankel_nufht.c:
+// 923-987) and nufht_batch dispatch (:989-1107) for every transform.
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+
+using namespace nufht_dev;
+
+// ---------------------------------------------------------------------------
+// errors
+static thread_local char g_err[1024] = "";
+
+extern "C" void nufht_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *nufht_amd_last_error(void) { return g_err; }
+extern "C" const char *nufht_amd_version(void) { return "nufht-amd 0.1 gfx950"; }
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            nufht_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),     \
+                            __FILE__, __LINE__);                                       \
+            return -510;                                                               \
+        }                                                                              \
+    } while (0)
+
+#define RC_TRY(expr)                   \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != 0) return rc_;      \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// per-stage timing
+enum Stage { ST_ZERO, ST_PERM, ST_DIR, ST_LOCP, ST_LOCE, ST_SPREAD, ST_FFT, ST_INTERP,
+             ST_GRAD, ST_COUNT };
+static const char *kStageNames[ST_COUNT] = {"zero", "permute", "direct", "loc_project",
+                                            "loc_eval", "asy_spread", "asy_fft",
+                                            "asy_interp", "grad_combine"};
+struct PendingEv { hipEvent_t a, b; int stage; };
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<PendingEv> g_prof_pending;
+static double g_prof_ms[ST_COUNT];
+static long long g_prof_n[ST_COUNT];
+
+struct StageTimer {
+    int stage;
+    hipStream_t st;
+    hipEvent_t a = nullptr;
+    StageTimer(int s, hipStream_t stream) : stage(s), st(stream)
+    {
+        if (g_prof_on) {
+            hipEventCreate(&a);
+            hipEventRecord(a, st);
+        }
+    }
+    ~StageTimer()
+    {
+        if (a) {
+            hipEvent_t b;
+            hipEventCreate(&b);
+            hipEventRecord(b, st);
+            std::lock_guard<std::mutex> g(g_prof_mu);
+            g_prof_pending.push_back({a, b, stage});
+        }
+    }
+};
+
+extern "C" int nufht_amd_profile(int enable)
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    g_prof_on = enable != 0;
+    return 0;
+}
+
+static void prof_drain()
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (auto &p : g_prof_pending) {
+        hipEventSynchronize(p.b);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        g_prof_ms[p.stage] += ms;
+        g_prof_n[p.stage] += 1;
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    g_prof_pending.clear();
+}
+
+extern "C" int nufht_amd_profile_read(double *ms, long long *launches, int max_stages,
+                                      const char **names)
+{
+    prof_drain();
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    int n = max_stages < ST_COUNT ? max_stages : ST_COUNT;
+    for (int i = 0; i < n; ++i) {
+        if (ms) ms[i] = g_prof_ms[i];
+        if (launches) launches[i] = g_prof_n[i];
+        if (names) names[i] = kStageNames[i];
+    }
+    return ST_COUNT;
+}
+
+extern "C" void nufht_amd_profile_reset(void)
+{
+    prof_drain();
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (int i = 0; i < ST_COUNT; ++i) { g_prof_ms[i] = 0.0; g_prof_n[i] = 0; }
+}
+
+// ---------------------------------------------------------------------------
+// ES kernel parameters and the table of its Fourier transform
+struct KernelTable {
+    int kw;
+    double beta, kmax;
+    std::vector<double> coef;   // kNFT Chebyshev coefficients
+};
+
+static void gauss_legendre(int q, std::vector<double> &x, std::vector<double> &w)
+{
+    x.resize(q);
+    w.resize(q);
+    for (int i = 0; i < q; ++i) {
+        double z = std::cos(M_PI * (i + 0.75) / (q + 0.5)), p0 = 1, p1 = z, dp = 1;
+        for (int it = 0; it < 100; ++it) {
+            p0 = 1.0;
+            p1 = z;
+            for (int k = 2; k <= q; ++k) {
+                double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = q * (z * p1 - p0) / (z * z - 1.0);
+            double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        p0 = 1.0;
+        p1 = z;
+        for (int k = 2; k <= q; ++k) {
+            double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = q * (z * p1 - p0) / (z * z - 1.0);
+        x[i] = z;
+        w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+}
+
+static void es_params(double tol, int mixed, int *kw, double *beta)
+{
+    int w = (int)std::ceil(std::log10(1.0 / tol)) + (mixed ? 1 : 2);
+    w = std::max(4, std::min(16, w));
+    if (mixed) w = std::min(w, 9);
+    *kw = w;
+    *beta = 2.30 * w;
+}
+
+static const KernelTable &kernel_table(int kw, double beta)
+{
+    static std::mutex mu;
+    static std::vector<KernelTable *> cache;
+    std::lock_guard<std::mutex> g(mu);
+    for (auto *t : cache)
+        if (t->kw == kw && t->beta == beta) return *t;
+    KernelTable *t = new KernelTable();
+    t->kw = kw;
+    t->beta = beta;
+    t->kmax = kw * M_PI / 2.0;
+    std::vector<double> gx, gw;
+    gauss_legendre(200, gx, gw);
+    const int Q = 128;
+    std::vector<double> F(Q);
+    for (int q = 0; q < Q; ++q) {
+        double y = std::cos(M_PI * (q + 0.5) / Q);
+        double k = t->kmax * std::sqrt(std::max(0.0, (y + 1.0) / 2.0));
+        double s = 0.0;
+        for (size_t a = 0; a < gx.size(); ++a) {
+            double u = gx[a], tt = 1.0 - u * u;
+            double es = tt > 0 ? std::exp(beta * (std::sqrt(tt) - 1.0)) : 0.0;
+            s += gw[a] * es * std::cos(k * u);
+        }
+        F[q] = s;
+    }
+    t->coef.assign(kNFT, 0.0);
+    for (int n = 0; n < kNFT; ++n) {
+        double s = 0.0;
+        for (int q = 0; q < Q; ++q) s += F[q] * std::cos(n * M_PI * (q + 0.5) / Q);
+        t->coef[n] = s * (n == 0 ? 1.0 : 2.0) / Q;
+    }
+    cache.push_back(t);
+    return *t;
+}
+
+// FFT lengths restricted to 2^a and 3*2^a (prebuilt rocFFT kernels)
+static int fft_length(long long need)
+{
+    long long best = 1LL << 40;
+    for (long long p = 8; p < (1LL << 34); p <<= 1) {
+        if (p >= need) { best = std::min(best, p); break; }
+        if (3 * p >= need) best = std::min(best, 3 * p);
+    }
+    return (int)best;
+}
+
+// ---------------------------------------------------------------------------
+// per-plan device state
+struct DevPlan {
+    double *d_r = nullptr, *d_w = nullptr;
+    int *d_perm_r = nullptr, *d_perm_w = nullptr;
+};
+
+struct FftClass {
+    int nf = 0;
+    long long batch = 0, off = 0;
+    rocfft_plan plan = nullptr;
+    rocfft_execution_info info = nullptr;
+    void *work = nullptr;
+    size_t work_bytes = 0;
+};
+
+struct Program {
+    int prec = 0;     // 0 FP64, 2 mixed
+    int emax = 0;
+    std::vector<TfDesc> tfs;
+    TfDesc *d_tfs = nullptr;
+    std::vector<ZeroItem> zero;
+    ZeroItem *d_zero = nullptr;
+    std::vector<DirItem> dir;
+    DirItem *d_dir = nullptr;
+    std::vector<LocBoxD> lbox;
+    LocBoxD *d_lbox = nullptr;
+    std::vector<LocItem> lproj, leval;
+    LocItem *d_lproj = nullptr, *d_leval = nullptr;
+    double *d_coef = nullptr;
+    long long ncoef = 0;
+    int loc_ch = 256, loc_tpb = 64, loc_stride = 0;
+    size_t loc_lds_p = 0, loc_lds_e = 0;
+    std::vector<AsyBoxD> abox;
+    AsyBoxD *d_abox = nullptr;
+    std::vector<AsyItem> aspread, ainterp;
+    AsyItem *d_aspread = nullptr, *d_ainterp = nullptr;
+    void *d_grid = nullptr;
+    long long grid_elems = 0;
+    std::vector<FftClass> fft;
+    double *d_ft = nullptr, *d_cheb = nullptr;
+    NufhtPlanStats stats;
+    bool desc_dirty = true;
+};
+
+struct EngineState {
+    std::mutex mu;
+    DevPlan dev;
+    bool dev_ready = false;
+    Program *prog = nullptr;
+    int prog_B = 0;
+    int prog_prec = -1;
+    // staging for host-pointer calls and permutations
+    double *d_in = nullptr, *d_out = nullptr;
+    size_t in_cap = 0, out_cap = 0;
+    double *d_tmp[6] = {nullptr};
+    size_t tmp_cap[6] = {0};
+};
+
+static int ensure_buf(double **p, size_t *cap, size_t need)
+{
+    if (*cap >= need && *p) return 0;
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    HIP_TRY(hipMalloc((void **)p, std::max<size_t>(need, 1) * sizeof(double)));
+    *cap = need;
+    return 0;
+}
+
+template <typename T>
+static int upload(T **dptr, const std::vector<T> &v)
+{
+    if (*dptr) hipFree(*dptr);
+    *dptr = nullptr;
+    if (v.empty()) return 0;
+    HIP_TRY(hipMalloc((void **)dptr, v.size() * sizeof(T)));
+    HIP_TRY(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static void free_program(Program *P)
+{
+    if (!P) return;
+    hipFree(P->d_tfs); hipFree(P->d_zero); hipFree(P->d_dir); hipFree(P->d_lbox);
+    hipFree(P->d_lproj); hipFree(P->d_leval); hipFree(P->d_coef); hipFree(P->d_abox);
+    hipFree(P->d_aspread); hipFree(P->d_ainterp); hipFree(P->d_grid); hipFree(P->d_ft);
+    hipFree(P->d_cheb);
+    for (auto &c : P->fft) {
+        if (c.plan) rocfft_plan_destroy(c.plan);
+        if (c.info) rocfft_execution_info_destroy(c.info);
+        if (c.work) hipFree(c.work);
+    }
+    delete P;
+}
+
+static EngineState *engine_of(const NufhtPlan *plan)
+{
+    NufhtImplHost *h = (NufhtImplHost *)plan->impl;
+    if (!h->dev) h->dev = new EngineState();
+    return (EngineState *)h->dev;
+}
+
+static int ensure_dev(const NufhtPlan *plan, EngineState *E)
+{
+    if (E->dev_ready) return 0;
+    NufhtImplHost *h = (NufhtImplHost *)plan->impl;
+    HIP_TRY(hipMalloc((void **)&E->dev.d_r, sizeof(double) * plan->m));
+    HIP_TRY(hipMalloc((void **)&E->dev.d_w, sizeof(double) * plan->n));
+    HIP_TRY(hipMemcpy(E->dev.d_r, h->r, sizeof(double) * plan->m, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(E->dev.d_w, h->w, sizeof(double) * plan->n, hipMemcpyHostToDevice));
+    if (h->perm_r) {
+        HIP_TRY(hipMalloc((void **)&E->dev.d_perm_r, sizeof(int) * plan->m));
+        HIP_TRY(hipMemcpy(E->dev.d_perm_r, h->perm_r, sizeof(int) * plan->m, hipMemcpyHostToDevice));
+    }
+    if (h->perm_w) {
+        HIP_TRY(hipMalloc((void **)&E->dev.d_perm_w, sizeof(int) * plan->n));
+        HIP_TRY(hipMemcpy(E->dev.d_perm_w, h->perm_w, sizeof(int) * plan->n, hipMemcpyHostToDevice));
+    }
+    E->dev_ready = true;
+    return 0;
+}
+
+extern "C" void nufht_engine_release(NufhtImplHost *h)
+{
+    EngineState *E = (EngineState *)h->dev;
+    if (!E) return;
+    free_program(E->prog);
+    hipFree(E->dev.d_r); hipFree(E->dev.d_w); hipFree(E->dev.d_perm_r); hipFree(E->dev.d_perm_w);
+    hipFree(E->d_in); hipFree(E->d_out);
+    for (int i = 0; i < 6; ++i) hipFree(E->d_tmp[i]);
+    delete E;
+    h->dev = nullptr;
+}
+
+extern "C" void nufht_engine_scratch_release(NufhtScratch *s) { (void)s; }
+
+static int effective_prec(const NufhtPlan *plan)
+{
+    NufhtImplHost *h = (NufhtImplHost *)plan->impl;
+    if (h->precision == NUFHT_PREC_MIXED) return 2;
+    if (h->precision == NUFHT_PREC_AUTO && plan->tol >= 1e-6) return 2;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// program construction
+struct TfInput { const NufhtPlan *plan; int B; };
+
+static void add_chunks_2d(std::vector<LocItem> &v, int box, int count, int chunk, int B)
+{
+    for (int c0 = 0; c0 < B; c0 += kLocBM)
+        for (int s = 0; s < count; s += chunk) v.push_back({box, s, std::min(chunk, count - s), c0});
+}
+
+static int build_program(Program *P, const std::vector<TfInput> &in)
+{
+    static std::once_flag fft_once;
+    std::call_once(fft_once, []() { rocfft_setup(); });
+    P->prec = 0;
+    for (auto &x : in) P->prec = std::max(P->prec, effective_prec(x.plan) == 2 ? 2 : 0);
+    // a program mixes transforms only at one grid precision: FP64 wins
+    for (auto &x : in)
+        if (effective_prec(x.plan) == 0) P->prec = 0;
+    bool mixed = P->prec == 2;
+    memset(&P->stats, 0, sizeof(P->stats));
+    std::vector<double> ft_all, cheb_all;
+    std::map<std::pair<int, double>, int> ft_off;
+    int kloc_max = 0, maxo_max = 0, e_max = 0;
+    struct AsyPending { AsyBoxD b; int tb; };
+    std::vector<AsyPending> apend;
+    for (size_t k = 0; k < in.size(); ++k) {
+        const NufhtPlan *plan = in[k].plan;
+        NufhtImplHost *h = (NufhtImplHost *)plan->impl;
+        EngineState *E = engine_of(plan);
+        int B = in[k].B;
+        TfDesc T;
+        memset(&T, 0, sizeof(T));
+        T.r = E->dev.d_r;
+        T.w = E->dev.d_w;
+        T.m = plan->m;
+        T.n = plan->n;
+        T.B = B;
+        double anu = std::fabs(plan->nu);
+        long twice = std::lround(2.0 * anu);
+        T.half = (int)(twice & 1);
+        T.order = (int)(twice / 2);
+        T.K_asy = h->exec_K_asy;
+        T.K_loc = h->exec_K_loc;
+        bool nu_int = !T.half;
+        T.sign = (plan->nu < 0 && nu_int && (std::llround(plan->nu) & 1)) ? -1.0 : 1.0;
+        for (int q = 0; q < kMaxA; ++q) T.a[q] = q < 64 ? plan->asy_coef[q] : 0.0;
+        es_params(plan->tol, mixed, &T.kw, &T.beta);
+        const KernelTable &kt = kernel_table(T.kw, T.beta);
+        auto key = std::make_pair(T.kw, T.beta);
+        if (!ft_off.count(key)) {
+            ft_off[key] = (int)ft_all.size();
+            ft_all.insert(ft_all.end(), kt.coef.begin(), kt.coef.end());
+        }
+        T.ft_off = ft_off[key];
+        T.kmax = kt.kmax;
+        int tf = (int)k;
+        // zero items
+        for (int t = 0; t < B; ++t)
+            for (int s = 0; s < plan->n; s += 8192)
+                P->zero.push_back({tf, t, s, std::min(8192, plan->n - s)});
+        // direct boxes
+        std::vector<Box> dirb, locb, asyb;
+        if (h->use_direct) {
+            dirb.push_back({0, plan->n - 1, 0, plan->m - 1});
+        } else {
+            const Boxes &bx = h->exec_boxes;
+            for (size_t q = 0; q < bx.dir.length; ++q) dirb.push_back(bx.dir.data[q]);
+            if (T.K_loc >= 0)
+                for (size_t q = 0; q < bx.loc.length; ++q) locb.push_back(bx.loc.data[q]);
+            for (size_t q = 0; q < bx.asy.length; ++q) asyb.push_back(bx.asy.data[q]);
+        }
+        for (auto &b : dirb) {
+            int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
+            if (ni <= 0 || nj <= 0) continue;
+            int tpr = ni >= 64 ? 4 : (ni >= 16 ? 16 : 64);
+            int rows = 256 / tpr;
+            for (int c0 = 0; c0 < B; c0 += kDirBM)
+                for (int i = 0; i < ni; i += rows)
+                    for (int j = 0; j < nj; j += kDirTJ)
+                        P->dir.push_back({tf, b.i0 + i, std::min(rows, ni - i), b.j0 + j,
+                                          std::min(kDirTJ, nj - j), tpr, c0, 0});
+            P->stats.pairs_dir += (long long)ni * nj;
+            P->stats.n_dir++;
+        }
+        if (!locb.empty()) {
+            kloc_max = std::max(kloc_max, T.K_loc);
+            int maxo = T.half ? T.K_loc : (T.order >> 1) + T.K_loc + (T.order & 1);
+            maxo_max = std::max(maxo_max, maxo);
+            if (T.half) {
+                int K = T.K_loc, N = 2 * K + 2, p = T.order & 1;
+                T.cheb_off = (int)cheb_all.size();
+                for (int l = 0; l <= K; ++l)
+                    for (int q = 0; q <= K; ++q) {
+                        double th = M_PI * (q + 0.5) / N;
+                        double v;
+                        if (l == 0 && p == 0) v = 2.0 / N;
+                        else v = (4.0 / N) * std::cos((2 * l + p) * th) / 2.0;
+                        cheb_all.push_back(v);
+                    }
+            }
+        }
+        for (auto &b : locb) {
+            int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
+            if (ni <= 0 || nj <= 0) continue;
+            LocBoxD L;
+            L.tf = tf; L.i0 = b.i0; L.ni = ni; L.j0 = b.j0; L.nj = nj;
+            L.coef_off = (int)P->ncoef;
+            L.r_end = h->r[b.j1];
+            P->ncoef += (long long)(T.K_loc + 1) * B;
+            P->lbox.push_back(L);
+            P->stats.n_loc++;
+            P->stats.loc_src += nj;
+            P->stats.loc_tgt += ni;
+        }
+        int E2 = 2 * (T.K_asy + 1);
+        int tb = 1;
+        while (tb < B && tb < 16) tb <<= 1;
+        for (auto &b : asyb) {
+            int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
+            if (ni <= 0 || nj <= 0) continue;
+            double r0 = h->r[b.j0], r1 = h->r[b.j1], w0 = h->w[b.i0], w1 = h->w[b.i1];
+            AsyBoxD A;
+            memset(&A, 0, sizeof(A));
+            A.tf = tf; A.i0 = b.i0; A.ni = ni; A.j0 = b.j0; A.nj = nj; A.E = E2;
+            A.C = 0.5 * (r0 + r1);
+            double X = 0.5 * (r1 - r0);
+            A.D = 0.5 * (w0 + w1);
+            double S = 0.5 * (w1 - w0);
+            double Smin = 1e-12 * (std::fabs(A.D) + 1.0);
+            if (S < Smin) S = Smin;
+            const double sigma = 2.0;
+            A.h = M_PI / (sigma * S);
+            A.L = (int)std::ceil(X / A.h) + T.kw / 2 + 2;
+            long long Ng = 2LL * A.L + 1;
+            A.Nf = fft_length(std::max<long long>((long long)(sigma * Ng), 2LL * T.kw + 8));
+            A.delta = 2.0 * M_PI / ((double)A.Nf * A.h);
+            A.a1 = 0.5 * T.kw * A.h;
+            A.a2 = 0.5 * T.kw * A.delta;
+            A.rho = r0;
+            e_max = std::max(e_max, E2);
+            apend.push_back({A, tb});
+            P->stats.n_asy++;
+            P->stats.asy_src += nj;
+            P->stats.asy_tgt += ni;
+            P->stats.asy_channels += (long long)E2 * B;
+        }
+        P->tfs.push_back(T);
+    }
+    // asymptotic boxes grouped by FFT length; classes laid out back to back
+    std::map<int, std::vector<int>> byNf;
+    for (size_t q = 0; q < apend.size(); ++q) byNf[apend[q].b.Nf].push_back((int)q);
+    long long off = 0;
+    for (auto &kv : byNf) {
+        FftClass c;
+        c.nf = kv.first;
+        c.off = off;
+        for (int q : kv.second) {
+            AsyBoxD &A = apend[q].b;
+            const TfDesc &T = P->tfs[A.tf];
+            A.goff = off;
+            long long ch = (long long)A.E * T.B;
+            off += ch * A.Nf;
+            c.batch += ch;
+        }
+        P->fft.push_back(c);
+    }
+    P->grid_elems = off;
+    for (auto &ap : apend) {
+        int box = (int)P->abox.size();
+        const AsyBoxD &A = ap.b;
+        const TfDesc &T = P->tfs[A.tf];
+        P->abox.push_back(A);
+        int tb = ap.tb, tp = 256 / tb;
+        for (int c0 = 0; c0 < T.B; c0 += tb) {
+            for (int s = 0; s < A.Nf; s += tp)
+                P->aspread.push_back({box, s, std::min(tp, A.Nf - s), c0, tb, 0});
+            for (int s = 0; s < A.ni; s += tp)
+                P->ainterp.push_back({box, s, std::min(tp, A.ni - s), c0, tb, 0});
+        }
+        P->stats.asy_grid += A.Nf;
+        P->stats.asy_grid_points += (long long)A.E * T.B * A.Nf;
+    }
+    P->stats.fft_classes = (long long)P->fft.size();
+    P->emax = e_max <= 6 ? 6 : e_max <= 10 ? 10 : e_max <= 14 ? 14 : 22;
+    // local-expansion launch geometry (LDS budget 64 KiB)
+    if (!P->lbox.empty()) {
+        int kp1 = kloc_max + 1;
+        P->loc_ch = std::max(16, std::min(256, 7680 / (kp1 + kLocBM)));
+        P->loc_lds_p = (size_t)(kp1 * P->loc_ch + P->loc_ch * kLocBM) * sizeof(double);
+        P->loc_stride = kp1 + std::max(maxo_max + 1, kp1);
+        int tpb = (int)((60000 / sizeof(double) - (size_t)kp1 * kLocBM) / P->loc_stride);
+        P->loc_tpb = std::max(1, std::min(64, tpb));
+        P->loc_lds_e = (size_t)(P->loc_tpb * P->loc_stride + kp1 * kLocBM) * sizeof(double);
+        for (size_t b = 0; b < P->lbox.size(); ++b) {
+            const TfDesc &T = P->tfs[P->lbox[b].tf];
+            add_chunks_2d(P->lproj, (int)b, P->lbox[b].nj, P->loc_ch, T.B);
+            add_chunks_2d(P->leval, (int)b, P->lbox[b].ni, P->loc_tpb, T.B);
+        }
+    }
+    if (!P->tfs.empty()) {
+        P->stats.kernel_width = P->tfs[0].kw;
+        P->stats.K_asy = P->tfs[0].K_asy;
+        P->stats.K_loc = P->tfs[0].K_loc;
+        P->stats.z_split = ((NufhtImplHost *)in[0].plan->impl)->exec_z;
+    }
+    P->stats.precision = P->prec;
+    // uploads
+    RC_TRY(upload(&P->d_zero, P->zero));
+    RC_TRY(upload(&P->d_dir, P->dir));
+    RC_TRY(upload(&P->d_lbox, P->lbox));
+    RC_TRY(upload(&P->d_lproj, P->lproj));
+    RC_TRY(upload(&P->d_leval, P->leval));
+    RC_TRY(upload(&P->d_abox, P->abox));
+    RC_TRY(upload(&P->d_aspread, P->aspread));
+    RC_TRY(upload(&P->d_ainterp, P->ainterp));
+    RC_TRY(upload(&P->d_ft, ft_all));
+    if (cheb_all.empty()) cheb_all.push_back(0.0);
+    RC_TRY(upload(&P->d_cheb, cheb_all));
+    if (P->ncoef) HIP_TRY(hipMalloc((void **)&P->d_coef, P->ncoef * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&P->d_tfs, std::max<size_t>(1, P->tfs.size()) * sizeof(TfDesc)));
+    size_t esz = mixed ? sizeof(float2) : sizeof(double2);
+    if (P->grid_elems) HIP_TRY(hipMalloc(&P->d_grid, P->grid_elems * esz));
+    for (auto &c : P->fft) {
+        size_t len = (size_t)c.nf;
+        rocfft_status s = rocfft_plan_create(&c.plan, rocfft_placement_inplace,
+                                             rocfft_transform_type_complex_inverse,
+                                             mixed ? rocfft_precision_single
+                                                   : rocfft_precision_double,
+                                             1, &len, (size_t)c.batch, nullptr);
+        if (s != rocfft_status_success) {
+            nufht_set_error("rocfft_plan_create(len=%d, batch=%lld) failed: %d", c.nf, c.batch, (int)s);
+            return -520;
+        }
+        rocfft_execution_info_create(&c.info);
+        rocfft_plan_get_work_buffer_size(c.plan, &c.work_bytes);
+        if (c.work_bytes) {
+            HIP_TRY(hipMalloc(&c.work, c.work_bytes));
+            rocfft_execution_info_set_work_buffer(c.info, c.work, c.work_bytes);
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// program execution
+template <typename R, int EM>
+static void launch_asy(Program *P, hipStream_t st)
+{
+    typedef typename Cplx<R>::T G;
+    {
+        StageTimer tm(ST_SPREAD, st);
+        hipLaunchKernelGGL((k_asy_spread<R, EM>), dim3((unsigned)P->aspread.size()), dim3(256), 0,
+                           st, P->d_tfs, P->d_abox, P->d_aspread, (G *)P->d_grid, P->d_ft);
+    }
+    {
+        StageTimer tm(ST_FFT, st);
+        for (auto &c : P->fft) {
+            rocfft_execution_info_set_stream(c.info, st);
+            void *buf = (void *)((G *)P->d_grid + c.off);
+            rocfft_execute(c.plan, &buf, nullptr, c.info);
+        }
+    }
+    {
+        StageTimer tm(ST_INTERP, st);
+        hipLaunchKernelGGL((k_asy_interp<R, EM>), dim3((unsigned)P->ainterp.size()), dim3(256), 0,
+                           st, P->d_tfs, P->d_abox, P->d_ainterp, (const G *)P->d_grid, P->d_ft);
+    }
+}
+
+template <typename R>
+static void launch_asy_e(Program *P, hipStream_t st)
+{
+    switch (P->emax) {
+    case 6: launch_asy<R, 6>(P, st); break;
+    case 10: launch_asy<R, 10>(P, st); break;
+    case 14: launch_asy<R, 14>(P, st); break;
+    default: launch_asy<R, 22>(P, st); break;
+    }
+}
+
+static int run_program(Program *P, hipStream_t st)
+{
+    if (P->desc_dirty) {
+        HIP_TRY(hipMemcpyAsync(P->d_tfs, P->tfs.data(), P->tfs.size() * sizeof(TfDesc),
+                               hipMemcpyHostToDevice, st));
+        P->desc_dirty = false;
+    }
+    if (!P->zero.empty()) {
+        StageTimer tm(ST_ZERO, st);
+        hipLaunchKernelGGL(k_zero, dim3((unsigned)P->zero.size()), dim3(256), 0, st, P->d_tfs,
+                           P->d_zero);
+        if (P->ncoef)
+            hipLaunchKernelGGL(k_fill0, dim3((unsigned)std::min<long long>(1024, (P->ncoef + 255) / 256)),
+                               dim3(256), 0, st, P->d_coef, P->ncoef);
+    }
+    if (!P->lproj.empty()) {
+        {
+            StageTimer tm(ST_LOCP, st);
+            hipLaunchKernelGGL(k_loc_proj, dim3((unsigned)P->lproj.size()), dim3(256),
+                               P->loc_lds_p, st, P->d_tfs, P->d_lbox, P->d_lproj, P->d_coef,
+                               P->loc_ch);
+        }
+        {
+            StageTimer tm(ST_LOCE, st);
+            hipLaunchKernelGGL(k_loc_eval, dim3((unsigned)P->leval.size()), dim3(P->loc_tpb),
+                               P->loc_lds_e, st, P->d_tfs, P->d_lbox, P->d_leval, P->d_coef,
+                               P->d_cheb, P->loc_stride);
+        }
+    }
+    if (!P->dir.empty()) {
+        StageTimer tm(ST_DIR, st);
+        hipLaunchKernelGGL(k_dir, dim3((unsigned)P->dir.size()), dim3(256), 0, st, P->d_tfs,
+                           P->d_dir);
+    }
+    if (!P->aspread.empty()) {
+        if (P->prec == 2) launch_asy_e<float>(P, st);
+        else launch_asy_e<double>(P, st);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static void set_io(Program *P, int k, const double *cs, long long ld_cs, double *gs, long long ld_gs)
+{
+    TfDesc &T = P->tfs[k];
+    if (T.cs != cs || T.gs != gs || T.ld_cs != ld_cs || T.ld_gs != ld_gs) {
+        T.cs = cs;
+        T.gs = gs;
+        T.ld_cs = ld_cs;
+        T.ld_gs = ld_gs;
+        P->desc_dirty = true;
+    }
+}
+
+static int get_program(const NufhtPlan *plan, EngineState *E, int B, Program **out)
+{
+    RC_TRY(ensure_dev(plan, E));
+    int prec = effective_prec(plan);
+    if (!E->prog || E->prog_B != B || E->prog_prec != prec) {
+        free_program(E->prog);
+        E->prog = nullptr;
+        Program *P = new Program();
+        std::vector<TfInput> in{{plan, B}};
+        int rc = build_program(P, in);
+        if (rc) { free_program(P); return rc; }
+        E->prog = P;
+        E->prog_B = B;
+        E->prog_prec = prec;
+    }
+    *out = E->prog;
+    return 0;
+}
+
+// Sorted-order execution on device buffers (rows in ascending grid order).
+static int exec_sorted(const NufhtPlan *plan, const double *d_cs, long long ld_cs, double *d_gs,
+                       long long ld_gs, int B, hipStream_t st)
+{
+    EngineState *E = engine_of(plan);
+    Program *P;
+    RC_TRY(get_program(plan, E, B, &P));
+    set_io(P, 0, d_cs, ld_cs, d_gs, ld_gs);
+    return run_program(P, st);
+}
+
+static int gather_rows(const double *src, long long ld_src, double *dst, long long ld_dst,
+                       const int *perm, int rows, int cols, hipStream_t st)
+{
+    long long tot = (long long)rows * cols;
+    if (tot == 0) return 0;
+    StageTimer tm(ST_PERM, st);
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, src,
+                       ld_src, dst, ld_dst, perm, rows, cols);
+    return 0;
+}
+
+static int scatter_rows(const double *src, long long ld_src, double *dst, long long ld_dst,
+                        const int *perm, int rows, int cols, hipStream_t st)
+{
+    long long tot = (long long)rows * cols;
+    if (tot == 0) return 0;
+    StageTimer tm(ST_PERM, st);
+    hipLaunchKernelGGL(k_scatter_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, src,
+                       ld_src, dst, ld_dst, perm, rows, cols);
+    return 0;
+}
+
+// Device-pointer execution in the caller's row order.
+static int exec_device(const NufhtPlan *plan, const double *d_cs, long long ld_cs, double *d_gs,
+                       long long ld_gs, int B, hipStream_t st)
+{
+    NufhtImplHost *h = (NufhtImplHost *)plan->impl;
+    EngineState *E = engine_of(plan);
+    RC_TRY(ensure_dev(plan, E));
+    const double *cs_use = d_cs;
+    long long ld_use = ld_cs;
+    if (h->perm_r) {
+        RC_TRY(ensure_buf(&E->d_in, &E->in_cap, (size_t)plan->m * B));
+        RC_TRY(gather_rows(d_cs, ld_cs, E->d_in, plan->m, E->dev.d_perm_r, plan->m, B, st));
+        cs_use = E->d_in;
+        ld_use = plan->m;
+    }
+    if (h->perm_w) {
+        RC_TRY(ensure_buf(&E->d_out, &E->out_cap, (size_t)plan->n * B));
+        RC_TRY(exec_sorted(plan, cs_use, ld_use, E->d_out, plan->n, B, st));
+        RC_TRY(scatter_rows(E->d_out, plan->n, d_gs, ld_gs, E->dev.d_perm_w, plan->n, B, st));
+        return 0;
+    }
+    return exec_sorted(plan, cs_use, ld_use, d_gs, ld_gs, B, st);
+}
+
+extern "C" int nufht_batch_device(const NufhtPlan *plan, const double *d_cs, int ld_cs, double *d_gs,
+                                  int ld_gs, int batch, void *stream)
+{
+    if (!plan || !plan->impl || !d_cs || !d_gs || batch <= 0) return -400;
+    if (ld_cs < plan->m || ld_gs < plan->n) return -401;
+    EngineState *E = engine_of(plan);
+    std::lock_guard<std::mutex> g(E->mu);
+    int rc = exec_device(plan, d_cs, ld_cs, d_gs, ld_gs, batch, (hipStream_t)stream);
+    if (g_prof_on) hipStreamSynchronize((hipStream_t)stream);
+    return rc;
+}
+
+extern "C" int nufht_engine_batch_host(const NufhtPlan *plan, const double *cs, int ld_cs, double *gs,
+                                       int ld_gs, int batch, NufhtScratch *scratch)
+{
+    (void)scratch;
+    NufhtImplHost *h = (NufhtImplHost *)plan->impl;
+    EngineState *E = engine_of(plan);
+    std::lock_guard<std::mutex> g(E->mu);
+    int m = plan->m, n = plan->n;
+    RC_TRY(ensure_dev(plan, E));
+    RC_TRY(ensure_buf(&E->d_in, &E->in_cap, (size_t)m * batch));
+    RC_TRY(ensure_buf(&E->d_out, &E->out_cap, (size_t)n * batch));
+    hipStream_t st = nullptr;
+    if (h->perm_r) {
+        std::vector<double> tmp((size_t)m * batch);
+        for (int t = 0; t < batch; ++t)
+            for (int j = 0; j < m; ++j) tmp[(size_t)t * m + j] = cs[(size_t)t * ld_cs + h->perm_r[j]];
+        HIP_TRY(hipMemcpy(E->d_in, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        HIP_TRY(hipMemcpy2D(E->d_in, (size_t)m * sizeof(double), cs, (size_t)ld_cs * sizeof(double),
+                            (size_t)m * sizeof(double), batch, hipMemcpyHostToDevice));
+    }
+    RC_TRY(exec_sorted(plan, E->d_in, m, E->d_out, n, batch, st));
+    if (h->perm_w) {
+        std::vector<double> tmp((size_t)n * batch);
+        HIP_TRY(hipMemcpy(tmp.data(), E->d_out, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int t = 0; t < batch; ++t)
+            for (int i = 0; i < n; ++i) gs[(size_t)t * ld_gs + h->perm_w[i]] = tmp[(size_t)t * n + i];
+    } else {
+        HIP_TRY(hipMemcpy2D(gs, (size_t)ld_gs * sizeof(double), E->d_out, (size_t)n * sizeof(double),
+                            (size_t)n * sizeof(double), batch, hipMemcpyDeviceToHost));
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// derived plans: adjoint and order+1 transforms (built on the sorted grids)
+static int derived(const NufhtPlan *plan, int which, NufhtPlan **out)
+{
+    NufhtImplHost *h = (NufhtImplHost *)plan->impl;
+    NufhtPlan **slot = which == 0 ? &h->derived.adjoint
+                     : which == 1 ? &h->derived.adjoint_up : &h->derived.forward_up;
+    if (!*slot) {
+        NufhtPlan *d = (NufhtPlan *)calloc(1, sizeof(NufhtPlan));
+        if (!d) return -501;
+        double nu = which == 0 ? plan->nu : std::fabs(plan->nu) + 1.0;
+        int rc = (which == 2) ? nufht_plan_init(d, nu, h->r, plan->m, h->w, plan->n, &h->opt)
+                              : nufht_plan_init(d, nu, h->w, plan->n, h->r, plan->m, &h->opt);
+        if (rc) { free(d); return rc; }
+        ((NufhtImplHost *)d->impl)->precision = h->precision;
+        *slot = d;
+    }
+    *out = *slot;
+    return 0;
+}
+
+extern "C" int nufht_adjoint_device(const NufhtPlan *plan, const double *d_ys, int ld_ys, double *d_xs,
+                                    int ld_xs, int batch, void *stream)
+{
+    if (!plan || !plan->impl || !d_ys || !d_xs || batch <= 0) return -400;
+    if (ld_ys < plan->n || ld_xs < plan->m) return -401;
+    NufhtImplHost *h = (NufhtImplHost *)plan->impl;
+    EngineState *E = engine_of(plan);
+    std::lock_guard<std::mutex> g(E->mu);
+    hipStream_t st = (hipStream_t)stream;
+    RC_TRY(ensure_dev(plan, E));
+    NufhtPlan *adj;
+    RC_TRY(derived(plan, 0, &adj));
+    EngineState *EA = engine_of(adj);
+    const double *ys = d_ys;
+    long long ldy = ld_ys;
+    if (h->perm_w) {
+        RC_TRY(ensure_buf(&E->d_tmp[0], &E->tmp_cap[0], (size_t)plan->n * batch));
+        RC_TRY(gather_rows(d_ys, ld_ys, E->d_tmp[0], plan->n, E->dev.d_perm_w, plan->n, batch, st));
+        ys = E->d_tmp[0];
+        ldy = plan->n;
+    }
+    int rc;
+    {
+        std::lock_guard<std::mutex> ga(EA->mu);
+        if (h->perm_r) {
+            RC_TRY(ensure_buf(&E->d_tmp[1], &E->tmp_cap[1], (size_t)plan->m * batch));
+            rc = exec_sorted(adj, ys, ldy, E->d_tmp[1], plan->m, batch, st);
+            if (!rc) rc = scatter_rows(E->d_tmp[1], plan->m, d_xs, ld_xs, E->dev.d_perm_r, plan->m, batch, st);
+        } else {
+            rc = exec_sorted(adj, ys, ldy, d_xs, ld_xs, batch, st);
+        }
+    }
+    if (g_prof_on) hipStreamSynchronize(st);
+    return rc;
+}
+
+extern "C" int nufht_adjoint(const NufhtPlan *plan, const double *ys, int ld_ys, double *xs, int ld_xs,
+                             int batch)
+{
+    if (!plan || !plan->impl || !ys || !xs || batch <= 0) return -400;
+    if (ld_ys < plan->n || ld_xs < plan->m) return -401;
+    double *dy = nullptr, *dx = nullptr;
+    HIP_TRY(hipMalloc((void **)&dy, sizeof(double) * plan->n * batch));
+    HIP_TRY(hipMalloc((void **)&dx, sizeof(double) * plan->m * batch));
+    int rc = 0;
+    if (hipMemcpy2D(dy, plan->n * sizeof(double), ys, (size_t)ld_ys * sizeof(double),
+                    plan->n * sizeof(double), batch, hipMemcpyHostToDevice) != hipSuccess) rc = -510;
+    if (!rc) rc = nufht_adjoint_device(plan, dy, plan->n, dx, plan->m, batch, nullptr);
+    if (!rc && hipMemcpy2D(xs, (size_t)ld_xs * sizeof(double), dx, plan->m * sizeof(double),
+                           plan->m * sizeof(double), batch, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = -510;
+    hipFree(dy);
+    hipFree(dx);
+    return rc;
+}
+
+// Node gradients via J'_nu(x) = (nu/x) J_nu(x) - J_{nu+1}(x), DLMF 10.6.2.
+extern "C" int nufht_grad_nodes_device(const NufhtPlan *plan, const double *d_cs, int ld_cs,
+                                       const double *d_rho, int ld_rho, int batch, double *d_grad_r,
+                                       double *d_grad_w, void *stream)
+{
+    if (!plan || !plan->impl || !d_cs || !d_rho || batch <= 0) return -400;
+    if (ld_cs < plan->m || ld_rho < plan->n) return -401;
+    NufhtImplHost *h = (NufhtImplHost *)plan->impl;
+    double anu = std::fabs(plan->nu);
+    if ((std::fabs(anu - std::round(anu)) < 1e-12 && anu + 1.0 > 200.0) ||
+        (std::fabs(anu - std::round(anu)) >= 1e-12 && anu + 1.0 > 9.5)) {
+        nufht_set_error("node gradient needs order |nu|+1 within the supported range");
+        return -2;
+    }
+    EngineState *E = engine_of(plan);
+    std::lock_guard<std::mutex> g(E->mu);
+    hipStream_t st = (hipStream_t)stream;
+    int m = plan->m, n = plan->n, B = batch;
+    RC_TRY(ensure_dev(plan, E));
+    double sgn = (plan->nu < 0 && std::fabs(plan->nu - std::round(plan->nu)) < 1e-12 &&
+                  (std::llround(plan->nu) & 1)) ? -1.0 : 1.0;
+    // sorted copies of cs (m x B) and rho (n x B)
+    RC_TRY(ensure_buf(&E->d_tmp[0], &E->tmp_cap[0], (size_t)m * B));
+    RC_TRY(ensure_buf(&E->d_tmp[1], &E->tmp_cap[1], (size_t)n * B));
+    double *cs_s = E->d_tmp[0], *rho_s = E->d_tmp[1];
+    RC_TRY(gather_rows(d_cs, ld_cs, cs_s, m, E->dev.d_perm_r, m, B, st));
+    RC_TRY(gather_rows(d_rho, ld_rho, rho_s, n, E->dev.d_perm_w, n, B, st));
+    size_t big = (size_t)std::max(m, n) * B;
+    RC_TRY(ensure_buf(&E->d_tmp[2], &E->tmp_cap[2], big));
+    RC_TRY(ensure_buf(&E->d_tmp[3], &E->tmp_cap[3], big));
+    RC_TRY(ensure_buf(&E->d_tmp[4], &E->tmp_cap[4], big));
+    RC_TRY(ensure_buf(&E->d_tmp[5], &E->tmp_cap[5], (size_t)std::max(m, n) + B));
+    double *P1 = E->d_tmp[2], *Q1 = E->d_tmp[3], *W = E->d_tmp[4], *aux = E->d_tmp[5];
+    double *lim = aux + std::max(m, n);
+    if (d_grad_r) {
+        NufhtPlan *adj, *adj_up;
+        RC_TRY(derived(plan, 0, &adj));
+        RC_TRY(derived(plan, 1, &adj_up));
+        RC_TRY(exec_sorted(adj, rho_s, n, P1, m, B, st));                 // s * H^T rho
+        long long tot = (long long)n * B;
+        hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, rho_s,
+                           (long long)n, W, (long long)n, E->dev.d_w, n, B);
+        RC_TRY(exec_sorted(adj_up, W, n, Q1, m, B, st));                  // H^T_{nu+1} (w rho)
+        const double *lim0 = nullptr;
+        if (h->r[0] == 0.0 && anu == 1.0) {
+            hipLaunchKernelGGL(k_colsum_weighted, dim3(B), dim3(256), 0, st, rho_s, (long long)n,
+                               E->dev.d_w, n, B, lim);
+            lim0 = lim;
+        }
+        {
+            StageTimer tm(ST_GRAD, st);
+            hipLaunchKernelGGL(k_grad_combine, dim3((m + 255) / 256), dim3(256), 0, st, cs_s,
+                               (long long)m, P1, (long long)m, Q1, (long long)m, E->dev.d_r, lim0, anu,
+                               sgn, m, B, aux);
+        }
+        RC_TRY(scatter_rows(aux, m, d_grad_r, m, E->dev.d_perm_r, m, 1, st));
+    }
+    if (d_grad_w) {
+        NufhtPlan *fwd_up;
+        RC_TRY(derived(plan, 2, &fwd_up));
+        RC_TRY(exec_sorted(plan, cs_s, m, P1, n, B, st));                 // s * H c
+        long long tot = (long long)m * B;
+        hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, cs_s,
+                           (long long)m, W, (long long)m, E->dev.d_r, m, B);
+        RC_TRY(exec_sorted(fwd_up, W, m, Q1, n, B, st));                  // H_{nu+1} (r c)
+        const double *lim0 = nullptr;
+        if (h->w[0] == 0.0 && anu == 1.0) {
+            hipLaunchKernelGGL(k_colsum_weighted, dim3(B), dim3(256), 0, st, cs_s, (long long)m,
+                               E->dev.d_r, m, B, lim);
+            lim0 = lim;
+        }
+        {
+            StageTimer tm(ST_GRAD, st);
+            hipLaunchKernelGGL(k_grad_combine, dim3((n + 255) / 256), dim3(256), 0, st, rho_s,
+                               (long long)n, P1, (long long)n, Q1, (long long)n, E->dev.d_w, lim0, anu,
+                               sgn, n, B, aux);
+        }
+        RC_TRY(scatter_rows(aux, n, d_grad_w, n, E->dev.d_perm_w, n, 1, st));
+    }
+    HIP_TRY(hipGetLastError());
+    if (g_prof_on) hipStreamSynchronize(st);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// precision, statistics
+extern "C" int nufht_amd_plan_set_precision(NufhtPlan *plan, int mode)
+{
+    if (!plan || !plan->impl || mode < 0 || mode > 2) return -400;
+    ((NufhtImplHost *)plan->impl)->precision = mode;
+    return 0;
+}
+
+extern "C" int nufht_amd_plan_precision(const NufhtPlan *plan)
+{
+    if (!plan || !plan->impl) return -400;
+    return effective_prec(plan);
+}
+
+extern "C" int nufht_amd_plan_stats(const NufhtPlan *plan, int batch, NufhtPlanStats *st)
+{
+    if (!plan || !plan->impl || !st || batch <= 0) return -400;
+    EngineState *E = engine_of(plan);
+    std::lock_guard<std::mutex> g(E->mu);
+    Program *P;
+    RC_TRY(get_program(plan, E, batch, &P));
+    *st = P->stats;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// uneven multi-transform batches
+struct NufhtMulti {
+    std::vector<const NufhtPlan *> plans;
+    std::vector<int> B;
+    Program *P = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<const void *> gkey;
+};
+
+extern "C" int nufht_multi_create(NufhtMulti **out, int count, const NufhtPlan *const *plans,
+                                  const int *batch)
+{
+    if (!out || count <= 0 || !plans || !batch) return -400;
+    NufhtMulti *mt = new NufhtMulti();
+    std::vector<TfInput> in;
+    for (int k = 0; k < count; ++k) {
+        if (!plans[k] || !plans[k]->impl || batch[k] <= 0) { delete mt; return -400; }
+        EngineState *E = engine_of(plans[k]);
+        int rc = ensure_dev(plans[k], E);
+        if (rc) { delete mt; return rc; }
+        mt->plans.push_back(plans[k]);
+        mt->B.push_back(batch[k]);
+        in.push_back({plans[k], batch[k]});
+    }
+    mt->P = new Program();
+    int rc = build_program(mt->P, in);
+    if (rc) { free_program(mt->P); delete mt; return rc; }
+    *out = mt;
+    return 0;
+}
+
+extern "C" int nufht_multi_execute(NufhtMulti *mt, const double *const *d_cs, const int *ld_cs,
+                                   double *const *d_gs, const int *ld_gs, void *stream, int use_graph)
+{
+    if (!mt || !d_cs || !d_gs || !ld_cs || !ld_gs) return -400;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<const void *> key;
+    for (size_t k = 0; k < mt->plans.size(); ++k) {
+        const NufhtPlan *p = mt->plans[k];
+        NufhtImplHost *h = (NufhtImplHost *)p->impl;
+        if (h->perm_r || h->perm_w) {
+            nufht_set_error("multi-transform batches need ascending grids");
+            return -402;
+        }
+        if (ld_cs[k] < p->m || ld_gs[k] < p->n) return -401;
+        set_io(mt->P, (int)k, d_cs[k], ld_cs[k], d_gs[k], ld_gs[k]);
+        key.push_back(d_cs[k]);
+        key.push_back(d_gs[k]);
+    }
+    if (!use_graph || g_prof_on) {
+        int rc = run_program(mt->P, st);
+        if (g_prof_on) hipStreamSynchronize(st);
+        return rc;
+    }
+    if (mt->P->desc_dirty) {
+        HIP_TRY(hipMemcpyAsync(mt->P->d_tfs, mt->P->tfs.data(), mt->P->tfs.size() * sizeof(TfDesc),
+                               hipMemcpyHostToDevice, st));
+        mt->P->desc_dirty = false;
+    }
+    if (!mt->gexec || key != mt->gkey) {
+        if (mt->gexec) { hipGraphExecDestroy(mt->gexec); mt->gexec = nullptr; }
+        if (mt->graph) { hipGraphDestroy(mt->graph); mt->graph = nullptr; }
+        hipStream_t cap;
+        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        int rc = run_program(mt->P, cap);
+        hipGraph_t gr = nullptr;
+        hipError_t ce = hipStreamEndCapture(cap, &gr);
+        hipStreamDestroy(cap);
+        if (rc) return rc;
+        if (ce != hipSuccess) {
+            nufht_set_error("graph capture failed: %s", hipGetErrorString(ce));
+            return -530;
+        }
+        HIP_TRY(hipGraphInstantiate(&mt->gexec, gr, nullptr, nullptr, 0));
+        mt->graph = gr;
+        mt->gkey = key;
+    }
+    HIP_TRY(hipGraphLaunch(mt->gexec, st));
+    return 0;
+}
+
+extern "C" void nufht_multi_free(NufhtMulti *mt)
+{
+    if (!mt) return;
+    if (mt->gexec) hipGraphExecDestroy(mt->gexec);
+    if (mt->graph) hipGraphDestroy(mt->graph);
+    free_program(mt->P);
+    delete mt;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: the knob exposed today is the asymptotic-stage precision at
+// tol >= 1e-6 (FP32 vs FP64 grids); candidates are timed on the device with
+// the plan's own geometry and the faster one is kept.
+extern "C" int nufht_amd_autotune(NufhtPlan *plan, int batch, void *stream)
+{
+    if (!plan || !plan->impl || batch <= 0) return -400;
+    NufhtImplHost *h = (NufhtImplHost *)plan->impl;
+    if (plan->tol < 1e-6) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    double *dc = nullptr, *dg = nullptr;
+    HIP_TRY(hipMalloc((void **)&dc, sizeof(double) * plan->m * batch));
+    HIP_TRY(hipMalloc((void **)&dg, sizeof(double) * plan->n * batch));
+    HIP_TRY(hipMemset(dc, 0, sizeof(double) * plan->m * batch));
+    int modes[2] = {NUFHT_PREC_FP64, NUFHT_PREC_MIXED};
+    float best = 1e30f;
+    int best_mode = h->precision;
+    for (int q = 0; q < 2; ++q) {
+        h->precision = modes[q];
+        hipEvent_t a, b;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        int rc = nufht_batch_device(plan, dc, plan->m, dg, plan->n, batch, st);  // warm-up / build
+        hipEventRecord(a, st);
+        for (int r = 0; r < 3 && !rc; ++r) rc = nufht_batch_device(plan, dc, plan->m, dg, plan->n, batch, st);
+        hipEventRecord(b, st);
+        hipEventSynchronize(b);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, a, b);
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        if (!rc && ms < best) { best = ms; best_mode = modes[q]; }
+    }
+    h->precision = best_mode;
+    hipFree(dc);
+    hipFree(dg);
+    return 0;
+}
