@@ -13,6 +13,7 @@
 #include <rocfft/rocfft.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -223,6 +224,9 @@ static const KernelTable &kernel_table(int kw, double beta)
     return *t;
 }
 
+// sources per spread item before a position tile is split across items
+static const int kSpreadSplit = 2048;
+
 // FFT lengths restricted to 2^a and 3*2^a (prebuilt rocFFT kernels)
 static int fft_length(long long need)
 {
@@ -269,8 +273,10 @@ struct Program {
     size_t loc_lds_p = 0, loc_lds_e = 0;
     std::vector<AsyBoxD> abox;
     AsyBoxD *d_abox = nullptr;
-    std::vector<AsyItem> aspread, ainterp;
-    AsyItem *d_aspread = nullptr, *d_ainterp = nullptr;
+    std::vector<AsyItem> aspread, ainterp, azero;
+    std::vector<int2> arange;
+    AsyItem *d_aspread = nullptr, *d_ainterp = nullptr, *d_azero = nullptr;
+    int2 *d_arange = nullptr;
     void *d_grid = nullptr;
     long long grid_elems = 0;
     std::vector<FftClass> fft;
@@ -321,6 +327,7 @@ static void free_program(Program *P)
     hipFree(P->d_tfs); hipFree(P->d_zero); hipFree(P->d_dir); hipFree(P->d_lbox);
     hipFree(P->d_lproj); hipFree(P->d_leval); hipFree(P->d_coef); hipFree(P->d_abox);
     hipFree(P->d_aspread); hipFree(P->d_ainterp); hipFree(P->d_grid); hipFree(P->d_ft);
+    hipFree(P->d_azero); hipFree(P->d_arange);
     hipFree(P->d_cheb);
     for (auto &c : P->fft) {
         if (c.plan) rocfft_plan_destroy(c.plan);
@@ -551,11 +558,37 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         int box = (int)P->abox.size();
         const AsyBoxD &A = ap.b;
         const TfDesc &T = P->tfs[A.tf];
+        const NufhtImplHost *h = (const NufhtImplHost *)in[A.tf].plan->impl;
         P->abox.push_back(A);
         int tb = ap.tb, tp = 256 / tb;
+        const double *rb = h->r + A.j0, *re = h->r + A.j0 + A.nj;
         for (int c0 = 0; c0 < T.B; c0 += tb) {
-            for (int s = 0; s < A.Nf; s += tp)
-                P->aspread.push_back({box, s, std::min(tp, A.Nf - s), c0, tb, 0});
+            for (int s = 0; s < A.Nf; s += tp) {
+                int np = std::min(tp, A.Nf - s);
+                int lmin = INT32_MAX, lmax = INT32_MIN;
+                for (int p = s; p < s + np; ++p) {
+                    int l = p <= A.L ? p : p - A.Nf;
+                    if (l < -A.L || l > A.L) continue;
+                    lmin = std::min(lmin, l);
+                    lmax = std::max(lmax, l);
+                }
+                int jlo = 0, jhi = 0;
+                if (lmin <= lmax) {
+                    double xlo = A.C + (lmin - 1) * A.h - A.a1, xhi = A.C + (lmax + 1) * A.h + A.a1;
+                    jlo = A.j0 + (int)(std::lower_bound(rb, re, xlo) - rb);
+                    jhi = A.j0 + (int)(std::upper_bound(rb, re, xhi) - rb);
+                }
+                if (jhi - jlo <= kSpreadSplit) {
+                    P->aspread.push_back({box, s, np, c0, tb, 0});
+                    P->arange.push_back(make_int2(jlo, jhi));
+                } else {
+                    P->azero.push_back({box, s, np, c0, tb, 0});
+                    for (int j = jlo; j < jhi; j += kSpreadSplit) {
+                        P->aspread.push_back({box, s, np, c0, tb, 1});
+                        P->arange.push_back(make_int2(j, std::min(jhi, j + kSpreadSplit)));
+                    }
+                }
+            }
             for (int s = 0; s < A.ni; s += tp)
                 P->ainterp.push_back({box, s, std::min(tp, A.ni - s), c0, tb, 0});
         }
@@ -595,6 +628,8 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     RC_TRY(upload(&P->d_abox, P->abox));
     RC_TRY(upload(&P->d_aspread, P->aspread));
     RC_TRY(upload(&P->d_ainterp, P->ainterp));
+    RC_TRY(upload(&P->d_azero, P->azero));
+    RC_TRY(upload(&P->d_arange, P->arange));
     RC_TRY(upload(&P->d_ft, ft_all));
     if (cheb_all.empty()) cheb_all.push_back(0.0);
     RC_TRY(upload(&P->d_cheb, cheb_all));
@@ -631,8 +666,12 @@ static void launch_asy(Program *P, hipStream_t st)
     typedef typename Cplx<R>::T G;
     {
         StageTimer tm(ST_SPREAD, st);
+        if (!P->azero.empty())
+            hipLaunchKernelGGL((k_grid_zero<R>), dim3((unsigned)P->azero.size()), dim3(256), 0, st,
+                               P->d_tfs, P->d_abox, P->d_azero, (G *)P->d_grid);
         hipLaunchKernelGGL((k_asy_spread<R, EM>), dim3((unsigned)P->aspread.size()), dim3(256), 0,
-                           st, P->d_tfs, P->d_abox, P->d_aspread, (G *)P->d_grid, P->d_ft);
+                           st, P->d_tfs, P->d_abox, P->d_aspread, P->d_arange, (G *)P->d_grid,
+                           P->d_ft);
     }
     {
         StageTimer tm(ST_FFT, st);

@@ -300,7 +300,9 @@ __global__ void k_loc_eval(const TfDesc *__restrict__ tfs, const LocBoxD *__rest
 // Source weights are normalised by rho = r_j0 (the box's smallest r):
 // u_e = c (rho/r)^e r^{-1/2}; the target side carries (rho w)^{-e} w^{-1/2}.
 
-constexpr int kSpreadCH = 256;
+constexpr int kSpreadQ = 128;      // sources staged per LDS pass
+constexpr int kTapMax = 17;        // kernel width + 1
+constexpr int kInterpWin = 64;     // grid positions staged per target tile
 
 template <typename R> struct Cplx;
 template <> struct Cplx<double> { typedef double2 T; };
@@ -309,20 +311,57 @@ template <> struct Cplx<float> { typedef float2 T; };
 __device__ __forceinline__ double es_eval(double u, double beta) { return es_kernel(u, beta); }
 __device__ __forceinline__ float es_eval(float u, float beta) { return es_kernel_f(u, beta); }
 
-// R is the arithmetic type of the spread (double, or float in mixed mode);
-// source phases e^{i D x} and the deconvolution factor are FP64 always.
+__device__ __forceinline__ void grid_add(double2 *p, double x, double y)
+{
+    atomicAdd(&p->x, x);
+    atomicAdd(&p->y, y);
+}
+__device__ __forceinline__ void grid_add(float2 *p, float x, float y)
+{
+    atomicAdd(&p->x, x);
+    atomicAdd(&p->y, y);
+}
+
+// Zero the (e, t) rows of one position tile that several spread items add into.
+template <typename R>
+__global__ void k_grid_zero(const TfDesc *__restrict__ tfs, const AsyBoxD *__restrict__ boxes,
+                            const AsyItem *__restrict__ items, typename Cplx<R>::T *__restrict__ grid)
+{
+    AsyItem it = items[blockIdx.x];
+    AsyBoxD bx = boxes[it.box];
+    const TfDesc &T = tfs[bx.tf];
+    int tb = it.tb;
+    int nrow = bx.E * tb;
+    for (int q = threadIdx.x; q < nrow * it.count; q += blockDim.x) {
+        int row = q / it.count, p = it.start + q % it.count;
+        int e = row / tb, t = it.col0 + row % tb;
+        if (t >= T.B) continue;
+        typename Cplx<R>::T z;
+        z.x = (R)0;
+        z.y = (R)0;
+        grid[bx.goff + (long long)(e * T.B + t) * bx.Nf + p] = z;
+    }
+}
+
+// R is the arithmetic type of the accumulation (double, or float in mixed
+// mode).  Source phases e^{i D x}, kernel offsets l h - x and the
+// deconvolution factor are FP64 always.  An item owns a tile of grid
+// positions, a column chunk and a source sub-range [jlo, jhi); items that
+// split a tile's sources (pad == 1) add atomically into a zeroed tile.
 template <typename R, int EMAX>
 __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ tfs,
                                                     const AsyBoxD *__restrict__ boxes,
                                                     const AsyItem *__restrict__ items,
+                                                    const int2 *__restrict__ ranges,
                                                     typename Cplx<R>::T *__restrict__ grid,
                                                     const double *__restrict__ ft)
 {
-    __shared__ R s_x[kSpreadCH], s_br[kSpreadCH], s_bi[kSpreadCH], s_ri[kSpreadCH];
-    __shared__ R s_c[kSpreadCH * 16];
-    __shared__ int s_lo[256], s_hi[256];
-    __shared__ int s_blo, s_bhi;
+    __shared__ R s_zr[kSpreadQ][kTapMax], s_zi[kSpreadQ][kTapMax];
+    __shared__ R s_c[kSpreadQ][16];
+    __shared__ R s_ri[kSpreadQ];
+    __shared__ int s_pf[kSpreadQ];
     AsyItem it = items[blockIdx.x];
+    int2 jr = ranges[blockIdx.x];
     AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
     int tb = it.tb;
@@ -331,61 +370,51 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
     int l = p <= bx.L ? p : p - bx.Nf;
     bool valid = (pl < it.count) && (l >= -bx.L) && (l <= bx.L);
     int t = it.col0 + tl;
-    bool tvalid = t < T.B;
-    double y = (double)l * bx.h;
-    if (threadIdx.x == 0) { s_blo = 0x7fffffff; s_bhi = -1; }
-    __syncthreads();
-    if (tl == 0) {
-        int lo = 0, hi = 0;
-        if (valid) {
-            // sources with |y - x_j| < a1, x_j = r_j - C
-            double rlo = bx.C + y - bx.a1, rhi = bx.C + y + bx.a1;
-            int a = bx.j0, b = bx.j0 + bx.nj;
-            while (a < b) { int mid = (a + b) >> 1; if (T.r[mid] > rlo) b = mid; else a = mid + 1; }
-            lo = a;
-            b = bx.j0 + bx.nj;
-            while (a < b) { int mid = (a + b) >> 1; if (T.r[mid] >= rhi) b = mid; else a = mid + 1; }
-            hi = a;
-            if (hi > lo) { atomicMin(&s_blo, lo); atomicMax(&s_bhi, hi); }
-        }
-        s_lo[pl] = lo;
-        s_hi[pl] = hi;
-    }
-    __syncthreads();
+    bool active = valid && (t < T.B);
+    int kw = T.kw;
+    double beta = T.beta;
+    double inv_a1 = 1.0 / bx.a1;
     R acc_r[EMAX], acc_i[EMAX];
 #pragma unroll
     for (int e = 0; e < EMAX; ++e) { acc_r[e] = (R)0; acc_i[e] = (R)0; }
-    int blo = s_blo, bhi = s_bhi;
-    int mylo = s_lo[pl], myhi = s_hi[pl];
     int E = bx.E;
-    R inv_a1 = (R)(1.0 / bx.a1), beta = (R)T.beta;
-    R yr = (R)y;
-    for (int c0 = blo; c0 < bhi; c0 += kSpreadCH) {
-        int cn = min(kSpreadCH, bhi - c0);
+    for (int c0 = jr.x; c0 < jr.y; c0 += kSpreadQ) {
+        int cn = min(kSpreadQ, jr.y - c0);
         for (int q = threadIdx.x; q < cn; q += 256) {
-            int j = c0 + q;
-            double rj = T.r[j];
+            double rj = T.r[c0 + q];
             double x = rj - bx.C;
-            double s, c;
-            sincos(bx.D * x, &s, &c);
+            double sn, cs;
+            sincos(bx.D * x, &sn, &cs);
             double rs = 1.0 / sqrt(rj);
-            s_x[q] = (R)x;
-            s_br[q] = (R)(c * rs);
-            s_bi[q] = (R)(s * rs);
+            double br = cs * rs, bi = sn * rs;
+            int lf = (int)ceil((x - bx.a1) / bx.h);
+            s_pf[q] = lf;
             s_ri[q] = (R)(bx.rho / rj);
+            for (int k = 0; k < kTapMax; ++k) {
+                double psi = 0.0;
+                if (k <= kw) psi = (double)es_eval((R)(((double)(lf + k) * bx.h - x) * inv_a1), (R)beta);
+                s_zr[q][k] = (R)(psi * br);
+                s_zi[q][k] = (R)(psi * bi);
+            }
         }
         for (int q = threadIdx.x; q < cn * tb; q += 256) {
             int jq = q / tb, tt = q % tb;
             int tc = it.col0 + tt;
-            s_c[jq * 16 + tt] = tc < T.B ? (R)T.cs[(long long)tc * T.ld_cs + c0 + jq] : (R)0;
+            s_c[jq][tt] = tc < T.B ? (R)T.cs[(long long)tc * T.ld_cs + c0 + jq] : (R)0;
         }
         __syncthreads();
-        if (valid && tvalid) {
-            int qa = max(mylo, c0) - c0, qb = min(myhi, c0 + cn) - c0;
+        if (active) {
+            // sources of this chunk whose taps reach l: lf in [l - kw, l] (lf ascending)
+            int a = 0, b = cn;
+            while (a < b) { int mid = (a + b) >> 1; if (s_pf[mid] >= l - kw) b = mid; else a = mid + 1; }
+            int qa = a;
+            b = cn;
+            while (a < b) { int mid = (a + b) >> 1; if (s_pf[mid] > l) b = mid; else a = mid + 1; }
+            int qb = a;
             for (int q = qa; q < qb; ++q) {
-                R u = (yr - s_x[q]) * inv_a1;
-                R psi = es_eval(u, beta) * s_c[q * 16 + tl];
-                R zr = psi * s_br[q], zi = psi * s_bi[q];
+                int k = l - s_pf[q];
+                R cv = s_c[q][tl];
+                R zr = s_zr[q][k] * cv, zi = s_zi[q][k] * cv;
                 R ri = s_ri[q];
 #pragma unroll
                 for (int e = 0; e < EMAX; ++e) {
@@ -400,15 +429,20 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
         }
         __syncthreads();
     }
-    if (pl < it.count && tvalid) {
+    if (pl < it.count && t < T.B) {
         double dec = 0.0;
-        if (valid) dec = 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, y * bx.a2, T.kmax));
+        if (valid) dec = 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)l * bx.h * bx.a2, T.kmax));
         for (int e = 0; e < E; ++e) {
-            long long at = bx.goff + ((long long)(e * T.B + t)) * bx.Nf + p;
-            typename Cplx<R>::T v;
-            v.x = (R)((double)acc_r[e] * dec);
-            v.y = (R)((double)acc_i[e] * dec);
-            grid[at] = v;
+            typename Cplx<R>::T *dst = grid + bx.goff + ((long long)(e * T.B + t)) * bx.Nf + p;
+            R vr = (R)((double)acc_r[e] * dec), vi = (R)((double)acc_i[e] * dec);
+            if (it.pad) {
+                if (valid) grid_add(dst, vr, vi);
+            } else {
+                typename Cplx<R>::T v;
+                v.x = vr;
+                v.y = vi;
+                *dst = v;
+            }
         }
     }
 }
@@ -416,7 +450,9 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
 // Interpolation, deconvolution, phase and the reference's output weights
 // (src/expansions.c:523-531, :549-557).  Kernel weights phi, the factor
 // e^{i (phi0 + w C)} h Delta / psihat(s) and the weight base are computed
-// once per target in FP64 and shared through LDS by the column lanes.
+// once per target in FP64 and shared through LDS by the column lanes; the
+// grid window a tile of (sorted) targets touches is staged in LDS per
+// exponent when it is at most kInterpWin positions wide.
 template <typename R, int EMAX>
 __global__ __launch_bounds__(256) void k_asy_interp(const TfDesc *__restrict__ tfs,
                                                     const AsyBoxD *__restrict__ boxes,
@@ -427,6 +463,7 @@ __global__ __launch_bounds__(256) void k_asy_interp(const TfDesc *__restrict__ t
     __shared__ R s_phi[256][16];
     __shared__ int s_m0[256];
     __shared__ double s_fr[256], s_fi[256], s_b0[256], s_rw[256];
+    __shared__ typename Cplx<R>::T s_g[16][kInterpWin];
     AsyItem it = items[blockIdx.x];
     AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
@@ -456,31 +493,59 @@ __global__ __launch_bounds__(256) void k_asy_interp(const TfDesc *__restrict__ t
         s_rw[ti] = 1.0 / (bx.rho * w);
     }
     __syncthreads();
+    int nvalid = min(it.count, ti_n);
+    int pw0 = s_m0[0];
+    int pw = s_m0[nvalid - 1] + kw - pw0;
+    bool use_lds = pw <= kInterpWin;
     int ti = threadIdx.x % ti_n, tl = threadIdx.x / ti_n;
     int t = it.col0 + tl;
-    if (ti >= it.count || t >= T.B) return;
-    int i = bx.i0 + it.start + ti;
-    int m0 = s_m0[ti];
-    double fr = s_fr[ti], fi = s_fi[ti];
-    double base = s_b0[ti], rw = s_rw[ti];
+    bool active = ti < it.count && t < T.B;
+    int m0 = active ? s_m0[ti] : 0;
+    double fr = active ? s_fr[ti] : 0.0, fi = active ? s_fi[ti] : 0.0;
+    double base = active ? s_b0[ti] : 0.0, rw = active ? s_rw[ti] : 0.0;
     double out = 0.0;
     for (int e = 0; e < E; ++e) {
-        const typename Cplx<R>::T *row = grid + bx.goff + (long long)(e * T.B + t) * bx.Nf;
-        R gr = (R)0, gi = (R)0;
-        for (int k = 0; k < kw; ++k) {
-            int m = m0 + k;
-            m = m < 0 ? m + bx.Nf : m;
-            typename Cplx<R>::T v = row[m];
-            R f = s_phi[ti][k];
-            gr += f * v.x;
-            gi += f * v.y;
+        if (use_lds) {
+            for (int q = threadIdx.x; q < pw * tb; q += 256) {
+                int tt = q / pw, pos = q % pw;
+                int tc = it.col0 + tt;
+                int m = pw0 + pos;
+                m = m < 0 ? m + bx.Nf : m;
+                if (tc < T.B) s_g[tt][pos] = grid[bx.goff + (long long)(e * T.B + tc) * bx.Nf + m];
+            }
+            __syncthreads();
         }
-        double vr = fr * (double)gr - fi * (double)gi, vi = fr * (double)gi + fi * (double)gr;
-        double A = (((e >> 1) & 1) ? -T.a[e] : T.a[e]) * base;
-        out = fma(A, (e & 1) ? -vi : vr, out);
-        base *= rw;
+        if (active) {
+            R gr = (R)0, gi = (R)0;
+            if (use_lds) {
+                const typename Cplx<R>::T *row = &s_g[tl][m0 - pw0];
+                for (int k = 0; k < kw; ++k) {
+                    R f = s_phi[ti][k];
+                    gr += f * row[k].x;
+                    gi += f * row[k].y;
+                }
+            } else {
+                const typename Cplx<R>::T *row = grid + bx.goff + (long long)(e * T.B + t) * bx.Nf;
+                for (int k = 0; k < kw; ++k) {
+                    int m = m0 + k;
+                    m = m < 0 ? m + bx.Nf : m;
+                    typename Cplx<R>::T v = row[m];
+                    R f = s_phi[ti][k];
+                    gr += f * v.x;
+                    gi += f * v.y;
+                }
+            }
+            double vr = fr * (double)gr - fi * (double)gi, vi = fr * (double)gi + fi * (double)gr;
+            double A = (((e >> 1) & 1) ? -T.a[e] : T.a[e]) * base;
+            out = fma(A, (e & 1) ? -vi : vr, out);
+            base *= rw;
+        }
+        if (use_lds) __syncthreads();
     }
-    atomicAdd(&T.gs[(long long)t * T.ld_gs + i], out);
+    if (active) {
+        int i = bx.i0 + it.start + ti;
+        atomicAdd(&T.gs[(long long)t * T.ld_gs + i], out);
+    }
 }
 
 // ------------------------------------------------------------------------
