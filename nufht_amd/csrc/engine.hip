@@ -226,6 +226,9 @@ static const KernelTable &kernel_table(int kw, double beta)
 
 // sources per spread item before a position tile is split across items
 static const int kSpreadSplit = 2048;
+static const int kDenseSplit = 4096;
+// sources per grid cell from which a box uses the split-K spread kernel
+static const double kDenseCell = 32.0;
 
 // FFT lengths restricted to 2^a and 3*2^a (prebuilt rocFFT kernels)
 static int fft_length(long long need)
@@ -273,10 +276,12 @@ struct Program {
     size_t loc_lds_p = 0, loc_lds_e = 0;
     std::vector<AsyBoxD> abox;
     AsyBoxD *d_abox = nullptr;
-    std::vector<AsyItem> aspread, ainterp, azero;
-    std::vector<int2> arange;
-    AsyItem *d_aspread = nullptr, *d_ainterp = nullptr, *d_azero = nullptr;
-    int2 *d_arange = nullptr;
+    std::vector<AsyItem> aspread, ainterp, azero, adense, aimf;
+    std::vector<int2> arange, adrange;
+    AsyItem *d_aspread = nullptr, *d_ainterp = nullptr, *d_azero = nullptr, *d_adense = nullptr;
+    AsyItem *d_aimf = nullptr;
+    int2 *d_arange = nullptr, *d_adrange = nullptr;
+    int interp_tmax = 16;
     void *d_grid = nullptr;
     long long grid_elems = 0;
     std::vector<FftClass> fft;
@@ -327,7 +332,8 @@ static void free_program(Program *P)
     hipFree(P->d_tfs); hipFree(P->d_zero); hipFree(P->d_dir); hipFree(P->d_lbox);
     hipFree(P->d_lproj); hipFree(P->d_leval); hipFree(P->d_coef); hipFree(P->d_abox);
     hipFree(P->d_aspread); hipFree(P->d_ainterp); hipFree(P->d_grid); hipFree(P->d_ft);
-    hipFree(P->d_azero); hipFree(P->d_arange);
+    hipFree(P->d_azero); hipFree(P->d_arange); hipFree(P->d_adense); hipFree(P->d_adrange);
+    hipFree(P->d_aimf);
     hipFree(P->d_cheb);
     for (auto &c : P->fft) {
         if (c.plan) rocfft_plan_destroy(c.plan);
@@ -560,11 +566,18 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         const TfDesc &T = P->tfs[A.tf];
         const NufhtImplHost *h = (const NufhtImplHost *)in[A.tf].plan->impl;
         P->abox.push_back(A);
-        int tb = ap.tb, tp = 256 / tb;
+        int tb = ap.tb, tp = 256 / tb;                 // interpolation targets per item
+        int sp = 256 / std::max(1, tb / 2);            // spread positions per item
         const double *rb = h->r + A.j0, *re = h->r + A.j0 + A.nj;
+        // source-dense boxes (many sources per grid cell) use the split-K kernel
+        bool dense = (double)A.nj >= kDenseCell * (2.0 * A.L + 1.0);
+        int step = dense ? kDenseP : sp;
+        int split = dense ? kDenseSplit : kSpreadSplit;
+        std::vector<AsyItem> &items = dense ? P->adense : P->aspread;
+        std::vector<int2> &ranges = dense ? P->adrange : P->arange;
         for (int c0 = 0; c0 < T.B; c0 += tb) {
-            for (int s = 0; s < A.Nf; s += tp) {
-                int np = std::min(tp, A.Nf - s);
+            for (int s = 0; s < A.Nf; s += step) {
+                int np = std::min(step, A.Nf - s);
                 int lmin = INT32_MAX, lmax = INT32_MIN;
                 for (int p = s; p < s + np; ++p) {
                     int l = p <= A.L ? p : p - A.Nf;
@@ -578,19 +591,39 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                     jlo = A.j0 + (int)(std::lower_bound(rb, re, xlo) - rb);
                     jhi = A.j0 + (int)(std::upper_bound(rb, re, xhi) - rb);
                 }
-                if (jhi - jlo <= kSpreadSplit) {
-                    P->aspread.push_back({box, s, np, c0, tb, 0});
-                    P->arange.push_back(make_int2(jlo, jhi));
+                if (jhi - jlo <= split) {
+                    items.push_back({box, s, np, c0, tb, 0});
+                    ranges.push_back(make_int2(jlo, jhi));
                 } else {
                     P->azero.push_back({box, s, np, c0, tb, 0});
-                    for (int j = jlo; j < jhi; j += kSpreadSplit) {
-                        P->aspread.push_back({box, s, np, c0, tb, 1});
-                        P->arange.push_back(make_int2(j, std::min(jhi, j + kSpreadSplit)));
+                    for (int j = jlo; j < jhi; j += split) {
+                        items.push_back({box, s, np, c0, tb, 1});
+                        ranges.push_back(make_int2(j, std::min(jhi, j + split)));
                     }
                 }
             }
-            for (int s = 0; s < A.ni; s += tp)
-                P->ainterp.push_back({box, s, std::min(tp, A.ni - s), c0, tb, 0});
+            // interpolation: 16-target tiles whose kernel supports fit a 16-position
+            // window go to the MFMA kernel (pad = window start); the rest are
+            // merged into VALU items of up to 256/tb targets
+            int run0 = -1;
+            auto flush = [&](int upto) {
+                for (int s = run0; run0 >= 0 && s < upto; s += tp)
+                    P->ainterp.push_back({box, s, std::min(tp, upto - s), c0, tb, 0});
+                run0 = -1;
+            };
+            for (int s = 0; s < A.ni; s += 16) {
+                int cnt = std::min(16, A.ni - s);
+                double sf = h->w[A.i0 + s] - A.D, sl = h->w[A.i0 + s + cnt - 1] - A.D;
+                int m0f = (int)std::ceil((sf - A.a2) / A.delta);
+                int m0l = (int)std::ceil((sl - A.a2) / A.delta);
+                if (m0l - m0f + T.kw + 2 <= 16) {
+                    flush(s);
+                    P->aimf.push_back({box, s, cnt, c0, tb, m0f - 1});
+                } else if (run0 < 0) {
+                    run0 = s;
+                }
+            }
+            flush(A.ni);
         }
         P->stats.asy_grid += A.Nf;
         P->stats.asy_grid_points += (long long)A.E * T.B * A.Nf;
@@ -630,6 +663,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     RC_TRY(upload(&P->d_ainterp, P->ainterp));
     RC_TRY(upload(&P->d_azero, P->azero));
     RC_TRY(upload(&P->d_arange, P->arange));
+    RC_TRY(upload(&P->d_adense, P->adense));
+    RC_TRY(upload(&P->d_aimf, P->aimf));
+    RC_TRY(upload(&P->d_adrange, P->adrange));
+    for (auto &it : P->ainterp) P->interp_tmax = std::max(P->interp_tmax, 256 / it.tb);
     RC_TRY(upload(&P->d_ft, ft_all));
     if (cheb_all.empty()) cheb_all.push_back(0.0);
     RC_TRY(upload(&P->d_cheb, cheb_all));
@@ -669,9 +706,14 @@ static void launch_asy(Program *P, hipStream_t st)
         if (!P->azero.empty())
             hipLaunchKernelGGL((k_grid_zero<R>), dim3((unsigned)P->azero.size()), dim3(256), 0, st,
                                P->d_tfs, P->d_abox, P->d_azero, (G *)P->d_grid);
-        hipLaunchKernelGGL((k_asy_spread<R, EM>), dim3((unsigned)P->aspread.size()), dim3(256), 0,
-                           st, P->d_tfs, P->d_abox, P->d_aspread, P->d_arange, (G *)P->d_grid,
-                           P->d_ft);
+        if (!P->aspread.empty())
+            hipLaunchKernelGGL((k_asy_spread<R, EM>), dim3((unsigned)P->aspread.size()), dim3(256),
+                               0, st, P->d_tfs, P->d_abox, P->d_aspread, P->d_arange,
+                               (G *)P->d_grid, P->d_ft);
+        if (!P->adense.empty())
+            hipLaunchKernelGGL((k_asy_spread_mfma<R, EM>), dim3((unsigned)P->adense.size()),
+                               dim3(64), 0, st, P->d_tfs, P->d_abox, P->d_adense, P->d_adrange,
+                               (G *)P->d_grid, P->d_ft);
     }
     {
         StageTimer tm(ST_FFT, st);
@@ -683,8 +725,15 @@ static void launch_asy(Program *P, hipStream_t st)
     }
     {
         StageTimer tm(ST_INTERP, st);
-        hipLaunchKernelGGL((k_asy_interp<R, EM>), dim3((unsigned)P->ainterp.size()), dim3(256), 0,
-                           st, P->d_tfs, P->d_abox, P->d_ainterp, (const G *)P->d_grid, P->d_ft);
+        int tmx = P->interp_tmax;
+        size_t lds = interp_lds_off(tmx, 2, sizeof(R)) * sizeof(double);
+        if (!P->ainterp.empty())
+            hipLaunchKernelGGL((k_asy_interp<R, EM>), dim3((unsigned)P->ainterp.size()), dim3(256),
+                               lds, st, P->d_tfs, P->d_abox, P->d_ainterp, (const G *)P->d_grid,
+                               P->d_ft, tmx);
+        if (!P->aimf.empty())
+            hipLaunchKernelGGL((k_asy_interp_mfma<R, EM>), dim3((unsigned)P->aimf.size()), dim3(64),
+                               0, st, P->d_tfs, P->d_abox, P->d_aimf, (const G *)P->d_grid, P->d_ft);
     }
 }
 
@@ -733,7 +782,7 @@ static int run_program(Program *P, hipStream_t st)
         hipLaunchKernelGGL(k_dir, dim3((unsigned)P->dir.size()), dim3(256), 0, st, P->d_tfs,
                            P->d_dir);
     }
-    if (!P->aspread.empty()) {
+    if (!P->aspread.empty() || !P->adense.empty()) {
         if (P->prec == 2) launch_asy_e<float>(P, st);
         else launch_asy_e<double>(P, st);
     }

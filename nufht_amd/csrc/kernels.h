@@ -346,8 +346,10 @@ __global__ void k_grid_zero(const TfDesc *__restrict__ tfs, const AsyBoxD *__res
 // R is the arithmetic type of the accumulation (double, or float in mixed
 // mode).  Source phases e^{i D x}, kernel offsets l h - x and the
 // deconvolution factor are FP64 always.  An item owns a tile of grid
-// positions, a column chunk and a source sub-range [jlo, jhi); items that
-// split a tile's sources (pad == 1) add atomically into a zeroed tile.
+// positions, a column chunk (tb columns) and a source sub-range [jlo, jhi);
+// items that split a tile's sources (pad == 1) add atomically into a zeroed
+// tile.  Each thread owns one position and two columns; the tb/2 lanes of a
+// position read the same tap table entries (LDS broadcast).
 template <typename R, int EMAX>
 __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ tfs,
                                                     const AsyBoxD *__restrict__ boxes,
@@ -356,8 +358,10 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
                                                     typename Cplx<R>::T *__restrict__ grid,
                                                     const double *__restrict__ ft)
 {
-    __shared__ R s_zr[kSpreadQ][kTapMax], s_zi[kSpreadQ][kTapMax];
-    __shared__ R s_c[kSpreadQ][16];
+    typedef typename Cplx<R>::T C2;
+    __shared__ C2 s_z[kSpreadQ][kTapMax];        // psi * e^{iDx} r^{-1/2}
+    __shared__ C2 s_c[kSpreadQ][8];              // column pairs
+    __shared__ double s_x[kSpreadQ], s_br[kSpreadQ], s_bi[kSpreadQ];
     __shared__ R s_ri[kSpreadQ];
     __shared__ int s_pf[kSpreadQ];
     AsyItem it = items[blockIdx.x];
@@ -365,18 +369,19 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
     AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
     int tb = it.tb;
-    int tl = threadIdx.x % tb, pl = threadIdx.x / tb;
+    int tl2 = tb > 1 ? tb >> 1 : 1;          // lanes per position
+    int tl = threadIdx.x % tl2, pl = threadIdx.x / tl2;
     int p = it.start + pl;
     int l = p <= bx.L ? p : p - bx.Nf;
     bool valid = (pl < it.count) && (l >= -bx.L) && (l <= bx.L);
-    int t = it.col0 + tl;
-    bool active = valid && (t < T.B);
+    int t0 = it.col0 + 2 * tl;
+    bool active = valid && (t0 < T.B);
     int kw = T.kw;
     double beta = T.beta;
     double inv_a1 = 1.0 / bx.a1;
-    R acc_r[EMAX], acc_i[EMAX];
+    R a0r[EMAX], a0i[EMAX], a1r[EMAX], a1i[EMAX];
 #pragma unroll
-    for (int e = 0; e < EMAX; ++e) { acc_r[e] = (R)0; acc_i[e] = (R)0; }
+    for (int e = 0; e < EMAX; ++e) { a0r[e] = a0i[e] = a1r[e] = a1i[e] = (R)0; }
     int E = bx.E;
     for (int c0 = jr.x; c0 < jr.y; c0 += kSpreadQ) {
         int cn = min(kSpreadQ, jr.y - c0);
@@ -386,21 +391,32 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
             double sn, cs;
             sincos(bx.D * x, &sn, &cs);
             double rs = 1.0 / sqrt(rj);
-            double br = cs * rs, bi = sn * rs;
-            int lf = (int)ceil((x - bx.a1) / bx.h);
-            s_pf[q] = lf;
+            s_x[q] = x;
+            s_br[q] = cs * rs;
+            s_bi[q] = sn * rs;
+            s_pf[q] = (int)ceil((x - bx.a1) / bx.h);
             s_ri[q] = (R)(bx.rho / rj);
-            for (int k = 0; k < kTapMax; ++k) {
-                double psi = 0.0;
-                if (k <= kw) psi = (double)es_eval((R)(((double)(lf + k) * bx.h - x) * inv_a1), (R)beta);
-                s_zr[q][k] = (R)(psi * br);
-                s_zi[q][k] = (R)(psi * bi);
-            }
         }
-        for (int q = threadIdx.x; q < cn * tb; q += 256) {
-            int jq = q / tb, tt = q % tb;
-            int tc = it.col0 + tt;
-            s_c[jq][tt] = tc < T.B ? (R)T.cs[(long long)tc * T.ld_cs + c0 + jq] : (R)0;
+        for (int q = threadIdx.x; q < cn * 8; q += 256) {
+            int jq = q >> 3, pr = q & 7;
+            int ta = it.col0 + 2 * pr, tb2 = ta + 1;
+            C2 v;
+            v.x = ta < T.B && 2 * pr < tb ? (R)T.cs[(long long)ta * T.ld_cs + c0 + jq] : (R)0;
+            v.y = tb2 < T.B && 2 * pr + 1 < tb ? (R)T.cs[(long long)tb2 * T.ld_cs + c0 + jq] : (R)0;
+            s_c[jq][pr] = v;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < cn * kTapMax; q += 256) {
+            int jq = q / kTapMax, k = q % kTapMax;
+            double psi = 0.0;
+            if (k <= kw) {
+                double u = ((double)(s_pf[jq] + k) * bx.h - s_x[jq]) * inv_a1;
+                psi = (double)es_eval((R)u, (R)beta);
+            }
+            C2 z;
+            z.x = (R)(psi * s_br[jq]);
+            z.y = (R)(psi * s_bi[jq]);
+            s_z[jq][k] = z;
         }
         __syncthreads();
         if (active) {
@@ -412,127 +428,586 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
             while (a < b) { int mid = (a + b) >> 1; if (s_pf[mid] > l) b = mid; else a = mid + 1; }
             int qb = a;
             for (int q = qa; q < qb; ++q) {
-                int k = l - s_pf[q];
-                R cv = s_c[q][tl];
-                R zr = s_zr[q][k] * cv, zi = s_zi[q][k] * cv;
+                C2 z = s_z[q][l - s_pf[q]];
+                C2 cc = s_c[q][tl];
                 R ri = s_ri[q];
+                R z0r = z.x * cc.x, z0i = z.y * cc.x, z1r = z.x * cc.y, z1i = z.y * cc.y;
+                R rp = (R)1;
 #pragma unroll
                 for (int e = 0; e < EMAX; ++e) {
                     if (e < E) {
-                        acc_r[e] += zr;
-                        acc_i[e] += zi;
-                        zr *= ri;
-                        zi *= ri;
+                        a0r[e] = fma(z0r, rp, a0r[e]);
+                        a0i[e] = fma(z0i, rp, a0i[e]);
+                        a1r[e] = fma(z1r, rp, a1r[e]);
+                        a1i[e] = fma(z1i, rp, a1i[e]);
+                        rp *= ri;
                     }
                 }
             }
         }
         __syncthreads();
     }
-    if (pl < it.count && t < T.B) {
+    if (pl < it.count && t0 < T.B) {
         double dec = 0.0;
         if (valid) dec = 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)l * bx.h * bx.a2, T.kmax));
+        bool two = (t0 + 1 < T.B) && (2 * tl + 1 < tb);
         for (int e = 0; e < E; ++e) {
-            typename Cplx<R>::T *dst = grid + bx.goff + ((long long)(e * T.B + t)) * bx.Nf + p;
-            R vr = (R)((double)acc_r[e] * dec), vi = (R)((double)acc_i[e] * dec);
+            C2 *d0 = grid + bx.goff + ((long long)(e * T.B + t0)) * bx.Nf + p;
+            R v0r = (R)((double)a0r[e] * dec), v0i = (R)((double)a0i[e] * dec);
+            R v1r = (R)((double)a1r[e] * dec), v1i = (R)((double)a1i[e] * dec);
             if (it.pad) {
-                if (valid) grid_add(dst, vr, vi);
+                if (valid) {
+                    grid_add(d0, v0r, v0i);
+                    if (two) grid_add(d0 + bx.Nf, v1r, v1i);
+                }
             } else {
-                typename Cplx<R>::T v;
-                v.x = vr;
-                v.y = vi;
-                *dst = v;
+                C2 v;
+                v.x = v0r;
+                v.y = v0i;
+                *d0 = v;
+                if (two) {
+                    v.x = v1r;
+                    v.y = v1i;
+                    d0[bx.Nf] = v;
+                }
             }
         }
     }
 }
 
-// Interpolation, deconvolution, phase and the reference's output weights
-// (src/expansions.c:523-531, :549-557).  Kernel weights phi, the factor
-// e^{i (phi0 + w C)} h Delta / psihat(s) and the weight base are computed
-// once per target in FP64 and shared through LDS by the column lanes; the
-// grid window a tile of (sorted) targets touches is staged in LDS per
-// exponent when it is at most kInterpWin positions wide.
+// Spread for source-dense boxes (many sources per grid cell): a wave owns
+// one grid position; its lanes are (column pair, source split), so every
+// lane works on every staged chunk; the splits are reduced with cross-lane
+// shuffles at the end.  Powers (rho/r)^e come from LDS so the FMAs of one
+// source are independent.
+constexpr int kDenseQ = 128;
+constexpr int kDenseP = 4;        // positions per block (one per wave)
+
 template <typename R, int EMAX>
-__global__ __launch_bounds__(256) void k_asy_interp(const TfDesc *__restrict__ tfs,
-                                                    const AsyBoxD *__restrict__ boxes,
-                                                    const AsyItem *__restrict__ items,
-                                                    const typename Cplx<R>::T *__restrict__ grid,
-                                                    const double *__restrict__ ft)
+__global__ __launch_bounds__(256) void k_asy_spread_dense(const TfDesc *__restrict__ tfs,
+                                                          const AsyBoxD *__restrict__ boxes,
+                                                          const AsyItem *__restrict__ items,
+                                                          const int2 *__restrict__ ranges,
+                                                          typename Cplx<R>::T *__restrict__ grid,
+                                                          const double *__restrict__ ft)
 {
-    __shared__ R s_phi[256][16];
-    __shared__ int s_m0[256];
-    __shared__ double s_fr[256], s_fi[256], s_b0[256], s_rw[256];
-    __shared__ typename Cplx<R>::T s_g[16][kInterpWin];
+    typedef typename Cplx<R>::T C2;
+    __shared__ C2 s_z[kDenseQ][kDenseP];
+    __shared__ C2 s_c[kDenseQ][8];
+    __shared__ R s_rp[kDenseQ][EMAX];
+    __shared__ double s_x[kDenseQ], s_br[kDenseQ], s_bi[kDenseQ];
+    __shared__ int s_pf[kDenseQ];
+    AsyItem it = items[blockIdx.x];
+    int2 jr = ranges[blockIdx.x];
+    AsyBoxD bx = boxes[it.box];
+    const TfDesc &T = tfs[bx.tf];
+    int tb = it.tb;
+    int npair = tb > 1 ? tb >> 1 : 1;
+    int nsplit = 64 / npair;
+    int lane = threadIdx.x & 63, pl = threadIdx.x >> 6;
+    int tl = lane % npair, sp = lane / npair;
+    int p = it.start + pl;
+    int l = p <= bx.L ? p : p - bx.Nf;
+    bool valid = (pl < it.count) && (l >= -bx.L) && (l <= bx.L);
+    int t0 = it.col0 + 2 * tl;
+    bool active = valid && (t0 < T.B);
+    int kw = T.kw, E = bx.E;
+    double inv_a1 = 1.0 / bx.a1;
+    R a0r[EMAX], a0i[EMAX], a1r[EMAX], a1i[EMAX];
+#pragma unroll
+    for (int e = 0; e < EMAX; ++e) { a0r[e] = a0i[e] = a1r[e] = a1i[e] = (R)0; }
+    for (int c0 = jr.x; c0 < jr.y; c0 += kDenseQ) {
+        int cn = min(kDenseQ, jr.y - c0);
+        for (int q = threadIdx.x; q < cn; q += 256) {
+            double rj = T.r[c0 + q];
+            double x = rj - bx.C;
+            double sn, cs;
+            sincos(bx.D * x, &sn, &cs);
+            double rs = 1.0 / sqrt(rj);
+            s_x[q] = x;
+            s_br[q] = cs * rs;
+            s_bi[q] = sn * rs;
+            s_pf[q] = (int)ceil((x - bx.a1) / bx.h);
+            double ri = bx.rho / rj, rp = 1.0;
+#pragma unroll
+            for (int e = 0; e < EMAX; ++e) {
+                s_rp[q][e] = (R)rp;
+                rp *= ri;
+            }
+        }
+        for (int q = threadIdx.x; q < cn * 8; q += 256) {
+            int jq = q >> 3, pr = q & 7;
+            int ta = it.col0 + 2 * pr, tb2 = ta + 1;
+            C2 v;
+            v.x = ta < T.B && 2 * pr < tb ? (R)T.cs[(long long)ta * T.ld_cs + c0 + jq] : (R)0;
+            v.y = tb2 < T.B && 2 * pr + 1 < tb ? (R)T.cs[(long long)tb2 * T.ld_cs + c0 + jq] : (R)0;
+            s_c[jq][pr] = v;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < cn * kDenseP; q += 256) {
+            int jq = q / kDenseP, pp = q % kDenseP;
+            int pq = it.start + pp;
+            int lq = pq <= bx.L ? pq : pq - bx.Nf;
+            double psi = 0.0;
+            int k = lq - s_pf[jq];
+            if (pp < it.count && k >= 0 && k <= kw && lq >= -bx.L && lq <= bx.L)
+                psi = (double)es_eval((R)(((double)lq * bx.h - s_x[jq]) * inv_a1), (R)T.beta);
+            C2 z;
+            z.x = (R)(psi * s_br[jq]);
+            z.y = (R)(psi * s_bi[jq]);
+            s_z[jq][pp] = z;
+        }
+        __syncthreads();
+        if (active) {
+            for (int q = sp; q < cn; q += nsplit) {
+                C2 z = s_z[q][pl];
+                C2 cc = s_c[q][tl];
+                R z0r = z.x * cc.x, z0i = z.y * cc.x, z1r = z.x * cc.y, z1i = z.y * cc.y;
+#pragma unroll
+                for (int e = 0; e < EMAX; ++e) {
+                    if (e < E) {
+                        R rp = s_rp[q][e];
+                        a0r[e] = fma(z0r, rp, a0r[e]);
+                        a0i[e] = fma(z0i, rp, a0i[e]);
+                        a1r[e] = fma(z1r, rp, a1r[e]);
+                        a1i[e] = fma(z1i, rp, a1i[e]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // reduce the source splits (lanes tl + npair * sp)
+    for (int off = npair; off < 64; off <<= 1) {
+#pragma unroll
+        for (int e = 0; e < EMAX; ++e) {
+            a0r[e] += __shfl_xor(a0r[e], off, 64);
+            a0i[e] += __shfl_xor(a0i[e], off, 64);
+            a1r[e] += __shfl_xor(a1r[e], off, 64);
+            a1i[e] += __shfl_xor(a1i[e], off, 64);
+        }
+    }
+    if (sp == 0 && pl < it.count && t0 < T.B) {
+        double dec = 0.0;
+        if (valid) dec = 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)l * bx.h * bx.a2, T.kmax));
+        bool two = (t0 + 1 < T.B) && (2 * tl + 1 < tb);
+        for (int e = 0; e < E; ++e) {
+            C2 *d0 = grid + bx.goff + ((long long)(e * T.B + t0)) * bx.Nf + p;
+            R v0r = (R)((double)a0r[e] * dec), v0i = (R)((double)a0i[e] * dec);
+            R v1r = (R)((double)a1r[e] * dec), v1i = (R)((double)a1i[e] * dec);
+            if (it.pad) {
+                if (valid) {
+                    grid_add(d0, v0r, v0i);
+                    if (two) grid_add(d0 + bx.Nf, v1r, v1i);
+                }
+            } else {
+                C2 v;
+                v.x = v0r;
+                v.y = v0i;
+                *d0 = v;
+                if (two) {
+                    v.x = v1r;
+                    v.y = v1i;
+                    d0[bx.Nf] = v;
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// MFMA formulations of the asymptotic stage for dense boxes.
+//
+// 16x16x4 matrix cores, one wave (64 lanes) per item.  Operand layouts of
+// both builtins (checked on gfx950): A[i][k] in lane i + 16 k, B[k][j] in
+// lane j + 16 k; the 16x16 result sits in 4 registers per lane, row
+// l/16 + 4 v (f64) or 4 (l/16) + v (f32), column l % 16.
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+
+template <typename R> struct Mfma;
+template <> struct Mfma<double> {
+    typedef mf_d4 V;
+    static __device__ __forceinline__ V mma(double a, double b, V c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <> struct Mfma<float> {
+    typedef mf_f4 V;
+    static __device__ __forceinline__ V mma(float a, float b, V c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+// Spread, source-dense boxes.  For an item of 4 grid positions, columns
+// [col0, col0 + tb) and a source range, with nsub = 16 / tb:
+//   G[p][e][t] = sum_q psi(l_p h - x_q) b_q (rho/r_q)^e c[q][t],  b = e^{iDx} r^{-1/2}
+// as matrix products over q: rows (p, e1) with e1 < 4, columns (t, e2)
+// with e2 < nsub, e = (g*4 + e1)*nsub + e2 for exponent group g; A carries
+// psi b rho^{(4g+e1) nsub}, B carries c rho^{e2}.  Re and Im are separate
+// products sharing B.
+constexpr int kMfQ = 32;          // sources staged per LDS pass
+
+template <typename R, int EMAX>
+__global__ __launch_bounds__(64) void k_asy_spread_mfma(const TfDesc *__restrict__ tfs,
+                                                        const AsyBoxD *__restrict__ boxes,
+                                                        const AsyItem *__restrict__ items,
+                                                        const int2 *__restrict__ ranges,
+                                                        typename Cplx<R>::T *__restrict__ grid,
+                                                        const double *__restrict__ ft)
+{
+    typedef typename Cplx<R>::T C2;
+    typedef typename Mfma<R>::V V;
+    constexpr int GMAX = (EMAX + 3) / 4;
+    constexpr int RPN = (EMAX > 16 ? EMAX : 16) + 1;   // powers used by A (< E) and B (< nsub)
+    __shared__ C2 s_z[kMfQ][4];
+    __shared__ R s_c[kMfQ][16];
+    __shared__ R s_rp[kMfQ][RPN];
+    __shared__ double s_x[kMfQ];
+    __shared__ int s_pf[kMfQ];
+    AsyItem it = items[blockIdx.x];
+    int2 jr = ranges[blockIdx.x];
+    AsyBoxD bx = boxes[it.box];
+    const TfDesc &T = tfs[bx.tf];
+    const int lane = threadIdx.x;
+    const int tb = it.tb, nsub = 16 / tb;
+    const int E = bx.E, kw = T.kw;
+    const int G = (E + 4 * nsub - 1) / (4 * nsub);
+    const double inv_a1 = 1.0 / bx.a1;
+    // this lane's operand roles
+    const int ka = lane >> 4;              // k index (source within a step)
+    const int ia = lane & 15;              // A row: (p, e1)
+    const int pa = ia >> 2, e1a = ia & 3;
+    const int jb = lane & 15;              // B column: (t, e2)
+    const int tbk = jb % tb, e2b = jb / tb;
+    const int tcol = it.col0 + tbk;
+    V acc_r[GMAX], acc_i[GMAX];
+#pragma unroll
+    for (int g = 0; g < GMAX; ++g)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { acc_r[g][v] = (R)0; acc_i[g][v] = (R)0; }
+    for (int c0 = jr.x; c0 < jr.y; c0 += kMfQ) {
+        int cn = min(kMfQ, jr.y - c0);
+        if (lane < kMfQ) {
+            int q = lane;
+            double rj = q < cn ? T.r[c0 + q] : bx.C;
+            double x = rj - bx.C;
+            double sn, cs;
+            sincos(bx.D * x, &sn, &cs);
+            double rs = q < cn ? 1.0 / sqrt(rj) : 0.0;
+            s_x[q] = x;
+            int lf = (int)ceil((x - bx.a1) / bx.h);
+            s_pf[q] = lf;
+            double ri = q < cn ? bx.rho / rj : 0.0, rp = 1.0;
+            for (int e = 0; e < RPN; ++e) {
+                s_rp[q][e] = (R)rp;
+                rp *= ri;
+            }
+            // kernel taps for the item's 4 positions, times the source phase
+            for (int pp = 0; pp < 4; ++pp) {
+                int pq = it.start + pp;
+                int lq = pq <= bx.L ? pq : pq - bx.Nf;
+                int k = lq - lf;
+                double psi = 0.0;
+                if (q < cn && pp < it.count && k >= 0 && k <= kw && lq >= -bx.L && lq <= bx.L)
+                    psi = (double)es_eval((R)(((double)lq * bx.h - x) * inv_a1), (R)T.beta);
+                C2 z;
+                z.x = (R)(psi * cs * rs);
+                z.y = (R)(psi * sn * rs);
+                s_z[q][pp] = z;
+            }
+        }
+        for (int q2 = lane; q2 < kMfQ * 16; q2 += 64) {
+            int q = q2 >> 4, tt = q2 & 15;
+            int tc = it.col0 + tt;
+            s_c[q][tt] = (q < cn && tt < tb && tc < T.B) ? (R)T.cs[(long long)tc * T.ld_cs + c0 + q] : (R)0;
+        }
+        __syncthreads();
+        int nsteps = (cn + 3) >> 2;
+        for (int st = 0; st < nsteps; ++st) {
+            int q = 4 * st + ka;
+            C2 z = s_z[q][pa];
+            R bval = s_c[q][tbk] * s_rp[q][e2b];
+#pragma unroll
+            for (int g = 0; g < GMAX; ++g) {
+                if (g < G) {
+                    int ea = (4 * g + e1a) * nsub;
+                    R rp = ea < E ? s_rp[q][ea] : (R)0;
+                    acc_r[g] = Mfma<R>::mma(z.x * rp, bval, acc_r[g]);
+                    acc_i[g] = Mfma<R>::mma(z.y * rp, bval, acc_i[g]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // result element (row i = (p, e1), column j = (t, e2)) of group g
+#pragma unroll
+    for (int g = 0; g < GMAX; ++g) {
+        if (g >= G) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int i = Mfma<R>::row(lane, v), j = lane & 15;
+            int p = i >> 2, e1 = i & 3, t = it.col0 + j % tb, e2 = j / tb;
+            int e = (4 * g + e1) * nsub + e2;
+            if (p >= it.count || e >= E || t >= T.B || j % tb >= tb) continue;
+            int pos = it.start + p;
+            int l = pos <= bx.L ? pos : pos - bx.Nf;
+            bool valid = l >= -bx.L && l <= bx.L;
+            double dec = valid ? 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)l * bx.h * bx.a2, T.kmax)) : 0.0;
+            C2 *d = grid + bx.goff + ((long long)(e * T.B + t)) * bx.Nf + pos;
+            R vr = (R)((double)acc_r[g][v] * dec), vi = (R)((double)acc_i[g][v] * dec);
+            if (it.pad) {
+                if (valid) grid_add(d, vr, vi);
+            } else {
+                C2 w;
+                w.x = vr;
+                w.y = vi;
+                *d = w;
+            }
+        }
+    }
+}
+
+// Interpolation, target-dense tiles: 16 consecutive targets whose kernel
+// supports all fall in a window of nwin <= 16 grid positions starting at
+// mw0 (item.pad holds mw0, item.tb the column chunk; the window length is
+// derived on the device).  For exponent group g (nsub = 16/tb exponents):
+//   D[i][(t, e2)] = sum_m phi(s_i - m Delta) grid[e][t][m],  e = g nsub + e2
+// with the phi matrix computed once per tile; the FP64 epilogue applies
+// e^{i(phi0 + w C)} h Delta / psihat(s) and the Hankel weights.
+template <typename R, int EMAX>
+__global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict__ tfs,
+                                                        const AsyBoxD *__restrict__ boxes,
+                                                        const AsyItem *__restrict__ items,
+                                                        const typename Cplx<R>::T *__restrict__ grid,
+                                                        const double *__restrict__ ft)
+{
+    typedef typename Cplx<R>::T C2;
+    typedef typename Mfma<R>::V V;
+    __shared__ double s_s[16], s_fr[16], s_fi[16], s_b0[16];
+    __shared__ double s_pw[16][EMAX];
+    __shared__ int s_m0[16];
     AsyItem it = items[blockIdx.x];
     AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
-    int tb = it.tb, ti_n = 256 / tb;
-    int E = bx.E, kw = T.kw;
-    if (threadIdx.x < ti_n && threadIdx.x < it.count) {
-        int ti = threadIdx.x;
-        int i = bx.i0 + it.start + ti;
-        double w = T.w[i];
+    const int lane = threadIdx.x;
+    const int tb = it.tb, nsub = 16 / tb;
+    const int E = bx.E, kw = T.kw;
+    const int mw0 = it.pad;
+    if (lane < 16) {
+        int ti = lane;
+        double w = T.w[bx.i0 + it.start + min(ti, it.count - 1)];
         double s = w - bx.D;
-        int m0 = (int)ceil((s - bx.a2) / bx.delta);
-        double inv_a2 = 1.0 / bx.a2;
-        for (int k = 0; k < kw; ++k)
-            s_phi[ti][k] = (R)es_kernel((s - (double)(m0 + k) * bx.delta) * inv_a2, T.beta);
-        s_m0[ti] = m0;
-        // e^{i (phi0 + w C)}, phi0 = -pi (nu/2 + 1/4) = -q pi / 4
+        s_s[ti] = s;
+        s_m0[ti] = (int)ceil((s - bx.a2) / bx.delta);
         double nu = (double)T.order + 0.5 * (double)T.half;
         double sp, cp;
         sincos(w * bx.C, &sp, &cp);
         int q = (int)lround(2.0 * nu + 1.0) & 7;
         double c0 = cos_qpi4(q), s0 = -cos_qpi4((q + 6) & 7);
-        double fr = cp * c0 - sp * s0, fi = sp * c0 + cp * s0;
         double amp = bx.h * bx.delta / (bx.a1 * ft_eval(ft + T.ft_off, s * bx.a1, T.kmax));
-        s_fr[ti] = fr * amp;
-        s_fi[ti] = fi * amp;
+        s_fr[ti] = (cp * c0 - sp * s0) * amp;
+        s_fi[ti] = (sp * c0 + cp * s0) * amp;
+        s_b0[ti] = ti < it.count ? T.sign * sqrt(2.0 / kPi) / sqrt(w) : 0.0;
+        double rw = 1.0 / (bx.rho * w), pw = 1.0;
+        for (int e = 0; e < EMAX; ++e) {
+            s_pw[ti][e] = pw;
+            pw *= rw;
+        }
+    }
+    __syncthreads();
+    int nwin = s_m0[it.count - 1] + kw - mw0;
+    int nsteps = min(4, (nwin + 3) >> 2);
+    // A operand (phi) for each K step, reused by every exponent group
+    const int ia = lane & 15, ka = lane >> 4;
+    R phiA[4];
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        int m = mw0 + 4 * st + ka;
+        double u = (s_s[ia] - (double)m * bx.delta) / bx.a2;
+        phiA[st] = (st < nsteps && ia < it.count) ? (R)es_kernel(u, T.beta) : (R)0;
+    }
+    const int jb = lane & 15, tbk = jb % tb, e2b = jb / tb;
+    const int tcol = it.col0 + tbk;
+    const bool colok = tcol < T.B;
+    double out[4] = {0.0, 0.0, 0.0, 0.0};
+    const int G = (E + nsub - 1) / nsub;
+    for (int g = 0; g < G; ++g) {
+        int eb = g * nsub + e2b;
+        bool eok = colok && eb < E;
+        const C2 *row = grid + bx.goff + (long long)(eb * T.B + tcol) * bx.Nf;
+        V dr, di;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { dr[v] = (R)0; di[v] = (R)0; }
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            if (st < nsteps) {
+                int m = mw0 + 4 * st + ka;
+                m = m < 0 ? m + bx.Nf : (m >= bx.Nf ? m - bx.Nf : m);
+                C2 b;
+                b.x = (R)0;
+                b.y = (R)0;
+                if (eok) b = row[m];
+                dr = Mfma<R>::mma(phiA[st], b.x, dr);
+                di = Mfma<R>::mma(phiA[st], b.y, di);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int i = Mfma<R>::row(lane, v);
+            if (!eok || i >= it.count) continue;
+            double gr = (double)dr[v], gi = (double)di[v];
+            double vr = s_fr[i] * gr - s_fi[i] * gi, vi = s_fr[i] * gi + s_fi[i] * gr;
+            double A = (((eb >> 1) & 1) ? -T.a[eb] : T.a[eb]) * s_b0[i] * s_pw[i][eb];
+            out[v] = fma(A, (eb & 1) ? -vi : vr, out[v]);
+        }
+    }
+    // fold the exponent sub-split (lanes j and j + tb * e2 share a column)
+    for (int off = tb; off < 16; off <<= 1)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[v] += __shfl_xor(out[v], off, 64);
+    if (e2b == 0 && colok) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int i = Mfma<R>::row(lane, v);
+            if (i < it.count) atomicAdd(&T.gs[(long long)tcol * T.ld_gs + bx.i0 + it.start + i], out[v]);
+        }
+    }
+}
+
+// Interpolation, deconvolution, phase and the reference's output weights
+// (src/expansions.c:523-531, :549-557).  Per-target factors (kernel weights
+// phi, e^{i (phi0 + w C)} h Delta / psihat(s), weight base) are computed in
+// FP64 by all lanes of the block together and shared through LDS; the grid
+// window a tile of (sorted) targets touches is staged in LDS for a group of
+// exponents at a time.  Lanes run over targets fastest, so lanes of one
+// column read neighbouring window entries.
+constexpr int kInterpLds = 20 * 1024;   // bytes of staged grid window
+
+// Offsets (in doubles) of the interpolation kernel's dynamic LDS regions:
+// part 0 = per-target scalars after phi[tmax][16]; part 1 = the grid window,
+// 16-byte aligned; part 2 = total size.
+__host__ __device__ inline size_t interp_lds_off(int tmax, int part, size_t rsize)
+{
+    size_t phi = ((size_t)tmax * 16 * rsize + 15) / 16 * 2;
+    if (part == 0) return phi;
+    size_t win = phi + 4 * (size_t)tmax + ((size_t)tmax + 1) / 2;
+    win = (win + 1) / 2 * 2;
+    if (part == 1) return win;
+    return win + kInterpLds / 8;
+}
+
+template <typename R, int EMAX>
+__global__ __launch_bounds__(256) void k_asy_interp(const TfDesc *__restrict__ tfs,
+                                                    const AsyBoxD *__restrict__ boxes,
+                                                    const AsyItem *__restrict__ items,
+                                                    const typename Cplx<R>::T *__restrict__ grid,
+                                                    const double *__restrict__ ft, int tmax)
+{
+    typedef typename Cplx<R>::T C2;
+    // dynamic LDS, sized per launch for the widest target tile (tmax):
+    // phi[tmax][16] | fr, fi, b0, rw [tmax] | m0[tmax] | grid window
+    extern __shared__ double s_dyn[];
+    AsyItem it = items[blockIdx.x];
+    AsyBoxD bx = boxes[it.box];
+    const TfDesc &T = tfs[bx.tf];
+    int tb = it.tb, ti_n = 256 / tb;
+    R(*s_phi)[16] = (R(*)[16])s_dyn;
+    double *s_fr = s_dyn + interp_lds_off(tmax, 0, sizeof(R));
+    double *s_fi = s_fr + tmax, *s_b0 = s_fi + tmax, *s_rw = s_b0 + tmax;
+    int *s_m0 = (int *)(s_rw + tmax);
+    C2 *s_g = (C2 *)(s_dyn + interp_lds_off(tmax, 1, sizeof(R)));
+    int E = bx.E, kw = T.kw;
+    int nt = min(it.count, ti_n);
+    double inv_a2 = 1.0 / bx.a2;
+    // per-target scalars: one thread per target
+    for (int ti = threadIdx.x; ti < nt; ti += 256) {
+        double w = T.w[bx.i0 + it.start + ti];
+        double s = w - bx.D;
+        s_m0[ti] = (int)ceil((s - bx.a2) / bx.delta);
+        double nu = (double)T.order + 0.5 * (double)T.half;
+        double sp, cp;
+        sincos(w * bx.C, &sp, &cp);
+        int q = (int)lround(2.0 * nu + 1.0) & 7;   // phi0 = -q pi / 4
+        double c0 = cos_qpi4(q), s0 = -cos_qpi4((q + 6) & 7);
+        double amp = bx.h * bx.delta / (bx.a1 * ft_eval(ft + T.ft_off, s * bx.a1, T.kmax));
+        s_fr[ti] = (cp * c0 - sp * s0) * amp;
+        s_fi[ti] = (sp * c0 + cp * s0) * amp;
         s_b0[ti] = T.sign * sqrt(2.0 / kPi) / sqrt(w);
         s_rw[ti] = 1.0 / (bx.rho * w);
     }
     __syncthreads();
-    int nvalid = min(it.count, ti_n);
+    // kernel weights: (target, tap) pairs over all lanes
+    for (int q = threadIdx.x; q < nt * kw; q += 256) {
+        int ti = q / kw, k = q % kw;
+        double s = T.w[bx.i0 + it.start + ti] - bx.D;
+        s_phi[ti][k] = (R)es_kernel((s - (double)(s_m0[ti] + k) * bx.delta) * inv_a2, T.beta);
+    }
     int pw0 = s_m0[0];
-    int pw = s_m0[nvalid - 1] + kw - pw0;
-    bool use_lds = pw <= kInterpWin;
+    int pw = s_m0[nt - 1] + kw - pw0;
+    int ge = kInterpLds / (int)sizeof(C2) / max(1, pw * tb);   // exponents staged per pass
+    bool use_lds = ge >= 1;
+    if (ge > E) ge = E;
+    __syncthreads();
     int ti = threadIdx.x % ti_n, tl = threadIdx.x / ti_n;
     int t = it.col0 + tl;
     bool active = ti < it.count && t < T.B;
     int m0 = active ? s_m0[ti] : 0;
     double fr = active ? s_fr[ti] : 0.0, fi = active ? s_fi[ti] : 0.0;
     double base = active ? s_b0[ti] : 0.0, rw = active ? s_rw[ti] : 0.0;
+    R phi[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) phi[k] = (active && k < kw) ? s_phi[ti][k] : (R)0;
     double out = 0.0;
-    for (int e = 0; e < E; ++e) {
-        if (use_lds) {
-            for (int q = threadIdx.x; q < pw * tb; q += 256) {
-                int tt = q / pw, pos = q % pw;
+    if (use_lds) {
+        for (int e0 = 0; e0 < E; e0 += ge) {
+            int ne = min(ge, E - e0);
+            for (int q = threadIdx.x; q < ne * tb * pw; q += 256) {
+                int pos = q % pw, row = q / pw;
+                int ee = row / tb, tt = row % tb;
                 int tc = it.col0 + tt;
                 int m = pw0 + pos;
                 m = m < 0 ? m + bx.Nf : m;
-                if (tc < T.B) s_g[tt][pos] = grid[bx.goff + (long long)(e * T.B + tc) * bx.Nf + m];
+                if (tc < T.B) s_g[q] = grid[bx.goff + (long long)((e0 + ee) * T.B + tc) * bx.Nf + m];
+            }
+            __syncthreads();
+            if (active) {
+                for (int ee = 0; ee < ne; ++ee) {
+                    const C2 *row = s_g + (ee * tb + tl) * pw + (m0 - pw0);
+                    R gr = (R)0, gi = (R)0;
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        if (k < kw) {
+                            C2 v = row[k];
+                            gr = fma(phi[k], v.x, gr);
+                            gi = fma(phi[k], v.y, gi);
+                        }
+                    }
+                    int e = e0 + ee;
+                    double vr = fr * (double)gr - fi * (double)gi, vi = fr * (double)gi + fi * (double)gr;
+                    double A = (((e >> 1) & 1) ? -T.a[e] : T.a[e]) * base;
+                    out = fma(A, (e & 1) ? -vi : vr, out);
+                    base *= rw;
+                }
             }
             __syncthreads();
         }
-        if (active) {
+    } else if (active) {
+        for (int e = 0; e < E; ++e) {
+            const C2 *row = grid + bx.goff + (long long)(e * T.B + t) * bx.Nf;
             R gr = (R)0, gi = (R)0;
-            if (use_lds) {
-                const typename Cplx<R>::T *row = &s_g[tl][m0 - pw0];
-                for (int k = 0; k < kw; ++k) {
-                    R f = s_phi[ti][k];
-                    gr += f * row[k].x;
-                    gi += f * row[k].y;
-                }
-            } else {
-                const typename Cplx<R>::T *row = grid + bx.goff + (long long)(e * T.B + t) * bx.Nf;
-                for (int k = 0; k < kw; ++k) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                if (k < kw) {
                     int m = m0 + k;
                     m = m < 0 ? m + bx.Nf : m;
-                    typename Cplx<R>::T v = row[m];
-                    R f = s_phi[ti][k];
-                    gr += f * v.x;
-                    gi += f * v.y;
+                    C2 v = row[m];
+                    gr = fma(phi[k], v.x, gr);
+                    gi = fma(phi[k], v.y, gi);
                 }
             }
             double vr = fr * (double)gr - fi * (double)gi, vi = fr * (double)gi + fi * (double)gr;
@@ -540,12 +1015,8 @@ __global__ __launch_bounds__(256) void k_asy_interp(const TfDesc *__restrict__ t
             out = fma(A, (e & 1) ? -vi : vr, out);
             base *= rw;
         }
-        if (use_lds) __syncthreads();
     }
-    if (active) {
-        int i = bx.i0 + it.start + ti;
-        atomicAdd(&T.gs[(long long)t * T.ld_gs + i], out);
-    }
+    if (active) atomicAdd(&T.gs[(long long)t * T.ld_gs + bx.i0 + it.start + ti], out);
 }
 
 // ------------------------------------------------------------------------

@@ -120,3 +120,28 @@ def test_unsorted_inputs_are_permuted(device):
     ref = oracle.direct(2.0, r, cs, k)
     assert rel_l2(g, ref[pk]) < 2e-8
     p.close()
+
+
+DENSE_CASES = [
+    # nu, m, n, R, K, B, tol, precision: wide r with narrow k (and the reverse)
+    # makes source-dense and target-dense asymptotic boxes (matrix-core paths)
+    (0.0, 40000, 30000, 1000.0, 40.0, 16, 1e-8, "fp64"),
+    (5.0, 30000, 40000, 40.0, 1000.0, 16, 1e-8, "fp64"),
+    (1.0, 40000, 40000, 500.0, 200.0, 8, 1e-10, "fp64"),
+    (2.0, 30000, 30000, 800.0, 60.0, 3, 1e-8, "fp64"),
+    (1.0, 40000, 30000, 1000.0, 50.0, 16, 1e-6, "mixed"),
+    (3.0, 30000, 30000, 300.0, 300.0, 5, 1e-6, "mixed"),
+]
+
+
+@pytest.mark.parametrize("nu,m,n,R,K,B,tol,prec", DENSE_CASES)
+def test_dense_boxes_against_direct_sum(device, nu, m, n, R, K, B, tol, prec):
+    r, k, cs = _uniform(int(m / 1000 + K), m, n, R, K, B)
+    say(f"dense nu={nu} m={m} n={n} R={R} K={K} B={B} tol={tol} {prec}")
+    p = _plan(nu, r, k, tol, precision=prec)
+    g = _fwd(p, cs, device)
+    rows = np.arange(0, n, max(1, n // 400))
+    ref = oracle.direct_rows(nu, r, cs, k, rows)
+    err = rel_l2(g[rows], ref)
+    assert err < 2 * tol, err
+    p.close()
