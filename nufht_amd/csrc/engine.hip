@@ -224,6 +224,9 @@ static const KernelTable &kernel_table(int kw, double beta)
     return *t;
 }
 
+// largest argument covered by a direct-sum Bessel table (unit intervals)
+static const int kJTabMax = 8192;
+
 // sources per spread item before a position tile is split across items
 static const int kSpreadSplit = 2048;
 static const int kDenseSplit = 4096;
@@ -285,7 +288,8 @@ struct Program {
     void *d_grid = nullptr;
     long long grid_elems = 0;
     std::vector<FftClass> fft;
-    double *d_ft = nullptr, *d_cheb = nullptr;
+    double *d_ft = nullptr, *d_cheb = nullptr, *d_jtab = nullptr;
+    long long jtab_total = 0;
     NufhtPlanStats stats;
     bool desc_dirty = true;
 };
@@ -335,6 +339,7 @@ static void free_program(Program *P)
     hipFree(P->d_azero); hipFree(P->d_arange); hipFree(P->d_adense); hipFree(P->d_adrange);
     hipFree(P->d_aimf);
     hipFree(P->d_cheb);
+    hipFree(P->d_jtab);
     for (auto &c : P->fft) {
         if (c.plan) rocfft_plan_destroy(c.plan);
         if (c.info) rocfft_execution_info_destroy(c.info);
@@ -464,6 +469,13 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 for (size_t q = 0; q < bx.loc.length; ++q) locb.push_back(bx.loc.data[q]);
             for (size_t q = 0; q < bx.asy.length; ++q) asyb.push_back(bx.asy.data[q]);
         }
+        // direct-sum Bessel table over [0, max w r of the direct boxes]
+        double xdir = 0.0;
+        for (auto &b : dirb)
+            if (b.i1 >= b.i0 && b.j1 >= b.j0) xdir = std::max(xdir, h->w[b.i1] * h->r[b.j1]);
+        T.jtab_n = xdir > 0.0 ? (int)std::min(std::ceil(xdir) + 1.0, (double)kJTabMax) : 0;
+        T.jtab_off = P->jtab_total;
+        P->jtab_total += (long long)T.jtab_n * (kJDeg + 1);
         for (auto &b : dirb) {
             int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
             if (ni <= 0 || nj <= 0) continue;
@@ -671,6 +683,12 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     if (cheb_all.empty()) cheb_all.push_back(0.0);
     RC_TRY(upload(&P->d_cheb, cheb_all));
     if (P->ncoef) HIP_TRY(hipMalloc((void **)&P->d_coef, P->ncoef * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&P->d_jtab, std::max<long long>(1, P->jtab_total) * sizeof(double)));
+    for (auto &T : P->tfs)
+        if (T.jtab_n > 0)
+            hipLaunchKernelGGL(k_jtab_build, dim3(T.jtab_n), dim3(kJDeg + 1), 0, nullptr, T.order,
+                               T.half, T.jtab_n, P->d_jtab + T.jtab_off);
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMalloc((void **)&P->d_tfs, std::max<size_t>(1, P->tfs.size()) * sizeof(TfDesc)));
     size_t esz = mixed ? sizeof(float2) : sizeof(double2);
     if (P->grid_elems) HIP_TRY(hipMalloc(&P->d_grid, P->grid_elems * esz));
@@ -780,7 +798,7 @@ static int run_program(Program *P, hipStream_t st)
     if (!P->dir.empty()) {
         StageTimer tm(ST_DIR, st);
         hipLaunchKernelGGL(k_dir, dim3((unsigned)P->dir.size()), dim3(256), 0, st, P->d_tfs,
-                           P->d_dir);
+                           P->d_dir, P->d_jtab);
     }
     if (!P->aspread.empty() || !P->adense.empty()) {
         if (P->prec == 2) launch_asy_e<float>(P, st);

@@ -14,6 +14,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "bessel_dev.h"
 
 namespace nufht_dev {
@@ -31,7 +33,8 @@ struct TfDesc {
     int kw;                // ES kernel width in grid points
     int ft_off;            // offset of the kernel-transform table in ft[]
     int cheb_off;          // half-integer local expansion matrix in cheb[]
-    int pad0;
+    int jtab_n;            // unit intervals of the direct-sum Bessel table (0: none)
+    long long jtab_off;    // offset of that table in jtab[]
     double beta;           // ES shape parameter
     double kmax;           // domain of the kernel-transform table
     double sign;           // -1 for negative odd integer orders
@@ -102,8 +105,46 @@ __global__ void k_fill0(double *__restrict__ p, long long n)
 constexpr int kDirTJ = 256;
 constexpr int kDirBM = 16;
 
+// J_nu on [0, jtab_n) as piecewise Chebyshev series of degree kJDeg on unit
+// intervals; half-integer orders tabulate J_nu(x) / sqrt(x), which is entire.
+constexpr int kJDeg = 15;
+
+__device__ __forceinline__ double jtab_eval(const double *__restrict__ tab, double x)
+{
+    int iv = (int)x;
+    double t = 2.0 * (x - (double)iv) - 1.0;
+    const double *c = tab + (long long)iv * (kJDeg + 1);
+    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int n = kJDeg; n >= 1; --n) {
+        double b0 = fma(2.0 * t, b1, c[n] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    return fma(t, b1, c[0] - b2);
+}
+
+// Table build: one block of kJDeg + 1 threads per unit interval; values of
+// the exact routine at Chebyshev nodes, then the discrete cosine transform.
+__global__ void k_jtab_build(int order, int half, int nint, double *__restrict__ tab)
+{
+    __shared__ double f[kJDeg + 1];
+    int iv = blockIdx.x, k = threadIdx.x;
+    constexpr int N = kJDeg + 1;
+    double tk = cos(kPi * ((double)k + 0.5) / N);
+    double x = (double)iv + 0.5 * (tk + 1.0);
+    double v = half ? jn_half(order, x, 0.0) : jn_int(order, x, 0.0);
+    if (half) v = x > 0.0 ? v / sqrt(x) : 0.0;
+    f[k] = v;
+    __syncthreads();
+    double acc = 0.0;
+    for (int q = 0; q < N; ++q) acc += f[q] * cos(kPi * (double)k * ((double)q + 0.5) / N);
+    tab[(long long)iv * N + k] = acc * (k == 0 ? 1.0 : 2.0) / N;
+}
+
 __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
-                                             const DirItem *__restrict__ items)
+                                             const DirItem *__restrict__ items,
+                                             const double *__restrict__ jtab)
 {
     __shared__ double s_r[kDirTJ];
     __shared__ double s_c[kDirTJ * kDirBM];
@@ -122,10 +163,19 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
     double acc[kDirBM];
 #pragma unroll
     for (int t = 0; t < kDirBM; ++t) acc[t] = 0.0;
+    const double *tab = jtab + T.jtab_off;
+    double xmax = (double)T.jtab_n;
     if (active) {
         double w = T.w[it.i0 + row];
         for (int j = lane; j < it.nj; j += tpr) {
-            double J = besselj_wr(T.order, T.half, w, s_r[j]);
+            double x = w * s_r[j];
+            double J;
+            if (x >= 0.0 && x < xmax) {
+                J = jtab_eval(tab, x);
+                if (T.half) J *= sqrt(x);
+            } else {
+                J = besselj_wr(T.order, T.half, w, s_r[j]);
+            }
 #pragma unroll
             for (int t = 0; t < kDirBM; ++t) acc[t] = fma(J, s_c[j * kDirBM + t], acc[t]);
         }
@@ -475,145 +525,8 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
     }
 }
 
-// Spread for source-dense boxes (many sources per grid cell): a wave owns
-// one grid position; its lanes are (column pair, source split), so every
-// lane works on every staged chunk; the splits are reduced with cross-lane
-// shuffles at the end.  Powers (rho/r)^e come from LDS so the FMAs of one
-// source are independent.
-constexpr int kDenseQ = 128;
-constexpr int kDenseP = 4;        // positions per block (one per wave)
-
-template <typename R, int EMAX>
-__global__ __launch_bounds__(256) void k_asy_spread_dense(const TfDesc *__restrict__ tfs,
-                                                          const AsyBoxD *__restrict__ boxes,
-                                                          const AsyItem *__restrict__ items,
-                                                          const int2 *__restrict__ ranges,
-                                                          typename Cplx<R>::T *__restrict__ grid,
-                                                          const double *__restrict__ ft)
-{
-    typedef typename Cplx<R>::T C2;
-    __shared__ C2 s_z[kDenseQ][kDenseP];
-    __shared__ C2 s_c[kDenseQ][8];
-    __shared__ R s_rp[kDenseQ][EMAX];
-    __shared__ double s_x[kDenseQ], s_br[kDenseQ], s_bi[kDenseQ];
-    __shared__ int s_pf[kDenseQ];
-    AsyItem it = items[blockIdx.x];
-    int2 jr = ranges[blockIdx.x];
-    AsyBoxD bx = boxes[it.box];
-    const TfDesc &T = tfs[bx.tf];
-    int tb = it.tb;
-    int npair = tb > 1 ? tb >> 1 : 1;
-    int nsplit = 64 / npair;
-    int lane = threadIdx.x & 63, pl = threadIdx.x >> 6;
-    int tl = lane % npair, sp = lane / npair;
-    int p = it.start + pl;
-    int l = p <= bx.L ? p : p - bx.Nf;
-    bool valid = (pl < it.count) && (l >= -bx.L) && (l <= bx.L);
-    int t0 = it.col0 + 2 * tl;
-    bool active = valid && (t0 < T.B);
-    int kw = T.kw, E = bx.E;
-    double inv_a1 = 1.0 / bx.a1;
-    R a0r[EMAX], a0i[EMAX], a1r[EMAX], a1i[EMAX];
-#pragma unroll
-    for (int e = 0; e < EMAX; ++e) { a0r[e] = a0i[e] = a1r[e] = a1i[e] = (R)0; }
-    for (int c0 = jr.x; c0 < jr.y; c0 += kDenseQ) {
-        int cn = min(kDenseQ, jr.y - c0);
-        for (int q = threadIdx.x; q < cn; q += 256) {
-            double rj = T.r[c0 + q];
-            double x = rj - bx.C;
-            double sn, cs;
-            sincos(bx.D * x, &sn, &cs);
-            double rs = 1.0 / sqrt(rj);
-            s_x[q] = x;
-            s_br[q] = cs * rs;
-            s_bi[q] = sn * rs;
-            s_pf[q] = (int)ceil((x - bx.a1) / bx.h);
-            double ri = bx.rho / rj, rp = 1.0;
-#pragma unroll
-            for (int e = 0; e < EMAX; ++e) {
-                s_rp[q][e] = (R)rp;
-                rp *= ri;
-            }
-        }
-        for (int q = threadIdx.x; q < cn * 8; q += 256) {
-            int jq = q >> 3, pr = q & 7;
-            int ta = it.col0 + 2 * pr, tb2 = ta + 1;
-            C2 v;
-            v.x = ta < T.B && 2 * pr < tb ? (R)T.cs[(long long)ta * T.ld_cs + c0 + jq] : (R)0;
-            v.y = tb2 < T.B && 2 * pr + 1 < tb ? (R)T.cs[(long long)tb2 * T.ld_cs + c0 + jq] : (R)0;
-            s_c[jq][pr] = v;
-        }
-        __syncthreads();
-        for (int q = threadIdx.x; q < cn * kDenseP; q += 256) {
-            int jq = q / kDenseP, pp = q % kDenseP;
-            int pq = it.start + pp;
-            int lq = pq <= bx.L ? pq : pq - bx.Nf;
-            double psi = 0.0;
-            int k = lq - s_pf[jq];
-            if (pp < it.count && k >= 0 && k <= kw && lq >= -bx.L && lq <= bx.L)
-                psi = (double)es_eval((R)(((double)lq * bx.h - s_x[jq]) * inv_a1), (R)T.beta);
-            C2 z;
-            z.x = (R)(psi * s_br[jq]);
-            z.y = (R)(psi * s_bi[jq]);
-            s_z[jq][pp] = z;
-        }
-        __syncthreads();
-        if (active) {
-            for (int q = sp; q < cn; q += nsplit) {
-                C2 z = s_z[q][pl];
-                C2 cc = s_c[q][tl];
-                R z0r = z.x * cc.x, z0i = z.y * cc.x, z1r = z.x * cc.y, z1i = z.y * cc.y;
-#pragma unroll
-                for (int e = 0; e < EMAX; ++e) {
-                    if (e < E) {
-                        R rp = s_rp[q][e];
-                        a0r[e] = fma(z0r, rp, a0r[e]);
-                        a0i[e] = fma(z0i, rp, a0i[e]);
-                        a1r[e] = fma(z1r, rp, a1r[e]);
-                        a1i[e] = fma(z1i, rp, a1i[e]);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-    // reduce the source splits (lanes tl + npair * sp)
-    for (int off = npair; off < 64; off <<= 1) {
-#pragma unroll
-        for (int e = 0; e < EMAX; ++e) {
-            a0r[e] += __shfl_xor(a0r[e], off, 64);
-            a0i[e] += __shfl_xor(a0i[e], off, 64);
-            a1r[e] += __shfl_xor(a1r[e], off, 64);
-            a1i[e] += __shfl_xor(a1i[e], off, 64);
-        }
-    }
-    if (sp == 0 && pl < it.count && t0 < T.B) {
-        double dec = 0.0;
-        if (valid) dec = 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)l * bx.h * bx.a2, T.kmax));
-        bool two = (t0 + 1 < T.B) && (2 * tl + 1 < tb);
-        for (int e = 0; e < E; ++e) {
-            C2 *d0 = grid + bx.goff + ((long long)(e * T.B + t0)) * bx.Nf + p;
-            R v0r = (R)((double)a0r[e] * dec), v0i = (R)((double)a0i[e] * dec);
-            R v1r = (R)((double)a1r[e] * dec), v1i = (R)((double)a1i[e] * dec);
-            if (it.pad) {
-                if (valid) {
-                    grid_add(d0, v0r, v0i);
-                    if (two) grid_add(d0 + bx.Nf, v1r, v1i);
-                }
-            } else {
-                C2 v;
-                v.x = v0r;
-                v.y = v0i;
-                *d0 = v;
-                if (two) {
-                    v.x = v1r;
-                    v.y = v1i;
-                    d0[bx.Nf] = v;
-                }
-            }
-        }
-    }
-}
+// Grid positions per tile of a source-dense (MFMA) spread item.
+constexpr int kDenseP = 4;
 
 // ------------------------------------------------------------------------
 // MFMA formulations of the asymptotic stage for dense boxes.
@@ -664,11 +577,11 @@ __global__ __launch_bounds__(64) void k_asy_spread_mfma(const TfDesc *__restrict
     typedef typename Mfma<R>::V V;
     constexpr int GMAX = (EMAX + 3) / 4;
     constexpr int RPN = (EMAX > 16 ? EMAX : 16) + 1;   // powers used by A (< E) and B (< nsub)
-    __shared__ C2 s_z[kMfQ][4];
+    constexpr int CPL = kMfQ * 16 / 64;                 // coefficients staged per lane
+    __shared__ R s_psi[kMfQ][4];                        // kernel taps at the item's positions
+    __shared__ C2 s_b[kMfQ];                            // e^{iDx} r^{-1/2}
     __shared__ R s_c[kMfQ][16];
     __shared__ R s_rp[kMfQ][RPN];
-    __shared__ double s_x[kMfQ];
-    __shared__ int s_pf[kMfQ];
     AsyItem it = items[blockIdx.x];
     int2 jr = ranges[blockIdx.x];
     AsyBoxD bx = boxes[it.box];
@@ -677,68 +590,96 @@ __global__ __launch_bounds__(64) void k_asy_spread_mfma(const TfDesc *__restrict
     const int tb = it.tb, nsub = 16 / tb;
     const int E = bx.E, kw = T.kw;
     const int G = (E + 4 * nsub - 1) / (4 * nsub);
+    const int npw = max(E, nsub);
     const double inv_a1 = 1.0 / bx.a1;
-    // this lane's operand roles
-    const int ka = lane >> 4;              // k index (source within a step)
-    const int ia = lane & 15;              // A row: (p, e1)
+    const int ka = lane >> 4;
+    const int ia = lane & 15;
     const int pa = ia >> 2, e1a = ia & 3;
-    const int jb = lane & 15;              // B column: (t, e2)
+    const int jb = lane & 15;
     const int tbk = jb % tb, e2b = jb / tb;
-    const int tcol = it.col0 + tbk;
+    // the item's grid positions (l values; out-of-range ones get no taps)
+    int lpos[4];
+#pragma unroll
+    for (int pp = 0; pp < 4; ++pp) {
+        int pq = it.start + pp;
+        int lq = pq <= bx.L ? pq : pq - bx.Nf;
+        lpos[pp] = (pp < it.count && lq >= -bx.L && lq <= bx.L) ? lq : INT_MIN / 2;
+    }
     V acc_r[GMAX], acc_i[GMAX];
 #pragma unroll
     for (int g = 0; g < GMAX; ++g)
 #pragma unroll
         for (int v = 0; v < 4; ++v) { acc_r[g][v] = (R)0; acc_i[g][v] = (R)0; }
+    // register prefetch of one chunk: lane q % kMfQ loads r; every lane CPL coefficients
+    const int qs = lane & (kMfQ - 1);
+    double r_pre;
+    R c_pre[CPL];
+    auto prefetch = [&](int c0) {
+        int cn = min(kMfQ, jr.y - c0);
+        r_pre = qs < cn ? T.r[c0 + qs] : bx.C;
+#pragma unroll
+        for (int u = 0; u < CPL; ++u) {
+            int q2 = lane + 64 * u, q = q2 >> 4, tt = q2 & 15;
+            int tc = it.col0 + tt;
+            c_pre[u] = (q < cn && tt < tb && tc < T.B) ? (R)T.cs[(long long)tc * T.ld_cs + c0 + q] : (R)0;
+        }
+    };
+    if (jr.x < jr.y) prefetch(jr.x);
     for (int c0 = jr.x; c0 < jr.y; c0 += kMfQ) {
         int cn = min(kMfQ, jr.y - c0);
-        if (lane < kMfQ) {
-            int q = lane;
-            double rj = q < cn ? T.r[c0 + q] : bx.C;
+        // stage the prefetched chunk: lanes < kMfQ take phases and powers,
+        // lanes >= kMfQ the kernel taps (same source q = lane % kMfQ)
+        {
+            int q = qs;
+            bool live = q < cn;
+            double rj = r_pre;
             double x = rj - bx.C;
-            double sn, cs;
-            sincos(bx.D * x, &sn, &cs);
-            double rs = q < cn ? 1.0 / sqrt(rj) : 0.0;
-            s_x[q] = x;
-            int lf = (int)ceil((x - bx.a1) / bx.h);
-            s_pf[q] = lf;
-            double ri = q < cn ? bx.rho / rj : 0.0, rp = 1.0;
-            for (int e = 0; e < RPN; ++e) {
-                s_rp[q][e] = (R)rp;
-                rp *= ri;
+            if (lane < kMfQ) {
+                double sn, cs;
+                sincos(bx.D * x, &sn, &cs);
+                double rs = live ? 1.0 / sqrt(rj) : 0.0;
+                C2 b;
+                b.x = (R)(cs * rs);
+                b.y = (R)(sn * rs);
+                s_b[q] = b;
+                double ri = live ? bx.rho / rj : 0.0, rp = 1.0;
+                for (int e = 0; e < npw; ++e) {
+                    s_rp[q][e] = (R)rp;
+                    rp *= ri;
+                }
+            } else {
+                int lf = (int)ceil((x - bx.a1) / bx.h);
+#pragma unroll
+                for (int pp = 0; pp < 4; ++pp) {
+                    int k = lpos[pp] - lf;
+                    R psi = (R)0;
+                    if (live && k >= 0 && k <= kw)
+                        psi = es_eval((R)(((double)lpos[pp] * bx.h - x) * inv_a1), (R)T.beta);
+                    s_psi[q][pp] = psi;
+                }
             }
-            // kernel taps for the item's 4 positions, times the source phase
-            for (int pp = 0; pp < 4; ++pp) {
-                int pq = it.start + pp;
-                int lq = pq <= bx.L ? pq : pq - bx.Nf;
-                int k = lq - lf;
-                double psi = 0.0;
-                if (q < cn && pp < it.count && k >= 0 && k <= kw && lq >= -bx.L && lq <= bx.L)
-                    psi = (double)es_eval((R)(((double)lq * bx.h - x) * inv_a1), (R)T.beta);
-                C2 z;
-                z.x = (R)(psi * cs * rs);
-                z.y = (R)(psi * sn * rs);
-                s_z[q][pp] = z;
+#pragma unroll
+            for (int u = 0; u < CPL; ++u) {
+                int q2 = lane + 64 * u;
+                s_c[q2 >> 4][q2 & 15] = c_pre[u];
             }
-        }
-        for (int q2 = lane; q2 < kMfQ * 16; q2 += 64) {
-            int q = q2 >> 4, tt = q2 & 15;
-            int tc = it.col0 + tt;
-            s_c[q][tt] = (q < cn && tt < tb && tc < T.B) ? (R)T.cs[(long long)tc * T.ld_cs + c0 + q] : (R)0;
         }
         __syncthreads();
+        if (c0 + kMfQ < jr.y) prefetch(c0 + kMfQ);   // in flight during the products
         int nsteps = (cn + 3) >> 2;
         for (int st = 0; st < nsteps; ++st) {
             int q = 4 * st + ka;
-            C2 z = s_z[q][pa];
+            R ps = s_psi[q][pa];
+            C2 b = s_b[q];
+            R zr = ps * b.x, zi = ps * b.y;
             R bval = s_c[q][tbk] * s_rp[q][e2b];
 #pragma unroll
             for (int g = 0; g < GMAX; ++g) {
                 if (g < G) {
                     int ea = (4 * g + e1a) * nsub;
                     R rp = ea < E ? s_rp[q][ea] : (R)0;
-                    acc_r[g] = Mfma<R>::mma(z.x * rp, bval, acc_r[g]);
-                    acc_i[g] = Mfma<R>::mma(z.y * rp, bval, acc_i[g]);
+                    acc_r[g] = Mfma<R>::mma(zr * rp, bval, acc_r[g]);
+                    acc_i[g] = Mfma<R>::mma(zi * rp, bval, acc_i[g]);
                 }
             }
         }
@@ -753,7 +694,7 @@ __global__ __launch_bounds__(64) void k_asy_spread_mfma(const TfDesc *__restrict
             int i = Mfma<R>::row(lane, v), j = lane & 15;
             int p = i >> 2, e1 = i & 3, t = it.col0 + j % tb, e2 = j / tb;
             int e = (4 * g + e1) * nsub + e2;
-            if (p >= it.count || e >= E || t >= T.B || j % tb >= tb) continue;
+            if (p >= it.count || e >= E || t >= T.B) continue;
             int pos = it.start + p;
             int l = pos <= bx.L ? pos : pos - bx.Nf;
             bool valid = l >= -bx.L && l <= bx.L;
