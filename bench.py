@@ -1,27 +1,39 @@
 """Benchmark: NUFHT column-points/s, forward + adjoint, on one MI355X.
 
-Workload (default): nu = 5, M = N = 1e6 nodes r, k ~ U[0, 1000) (sorted),
-B = 16 N(0,1) coefficient columns, tol 1e-8, FP64.  One step is one
-forward transform g = H c plus one adjoint x = H^T g, both on inputs
-already resident in HBM; column_points per step = 2 * B * (M + N).
+Headline workload: nu = 5, M = N = 1e6 nodes r, k ~ U[0, 1000) (sorted),
+B = 16 N(0,1) coefficient columns, tol 1e-8, FP64.  One step is one forward
+transform g = H c plus one adjoint x = H^T g on inputs already resident in
+HBM; column_points per step = 2 * B * (M + N).
 
-Printed: one JSON line with the contract fields plus
-  roofline      for the dominant kernel, timed with HIP events on its
-                launch stream inside the timed region;
-  cpu_baseline  the oracle's ref-algorithm CPU port (plan, boxes, Wimp
-                and Hankel expansions, its own CPU type-3 NUFFT) timed on
-                this host on a bounded sample of the same workload;
-  plain         the same GPU engine run one column per launch sequence
-                (the reference's nufht_batch_loop pattern), for the
-                batched-vs-plain comparison.
+The JSON line carries the contract fields plus
+  roofline      the dominant kernel (the stage with the most device time;
+                every stage but asy_fft is one kernel), timed with HIP
+                events on its launch stream inside the timed region, with
+                its algorithmic work per launch from the plan statistics;
+                traffic (HBM bytes per launch) is read from the committed
+                rocprofv3 FETCH_SIZE / WRITE_SIZE summary of that kernel
+                when one exists for this workload;
+  cpu_baseline  the oracle's CPU restatement of the reference algorithm
+                (boxes, Wimp and Hankel expansions, its own CPU type-3
+                NUFFT) on a bounded sample of the same workload, 1 thread;
+  plain         the same engine run one column at a time (the reference's
+                nufht_batch_loop pattern);
+  targets       the other workloads, each against its own comparator:
+                uneven 256-transform batch (one hipGraph-replayed launch
+                per stage vs a per-transform loop) at tol 1e-10 and 1e-6,
+                FP32-mixed vs FP64 at tol 1e-6 on the clustered nu = 1
+                workload, and the autograd forward + backward of the
+                gradient workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--no-targets] [--copy-calibration]
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -30,60 +42,107 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix peaks are equal)
+PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64; vector and matrix peaks are equal
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+METRIC = "NUFHT column-points/s at tol 1e-8/1e-6, batched vs plain GPU; fwd+adjoint"
 
 
-def stage_work(stats: dict, B: int) -> dict:
-    """Algorithmic work per launch of each stage (flops), from plan stats.
+def say(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
-    spread:  each asymptotic source adds kw complex contributions to each of
-             E*B channels: 4 flops (2 FMA) per (source, tap, channel)
-    interp:  each asymptotic target reads kw complex values per channel:
-             4 flops per (target, tap, channel)
-    direct:  2*B flops per pair (Bessel evaluation not counted)
+
+def stage_work(st: dict, B: int) -> dict:
+    """Algorithmic work of one launch of each single-kernel stage.
+
+    A channel is one complex series (exponent e, column t).  Spreading a
+    source into one channel is kernel_width complex multiply-adds of a real
+    kernel value (4 flops each); interpolating a target from one channel is
+    the same.  Direct sums: 2 B flops per (target, source) pair, the Bessel
+    evaluation not counted.  FFT: one read and one write of every complex
+    grid point of FP64 (16 B) or FP32 (8 B) per pass, two passes counted.
     """
-    kw = stats["kernel_width"]
-    ch_per_box = stats["asy_channels"] / max(1, stats["n_asy"])
-    E = 2 * (stats["K_asy"] + 1)
+    kw = st["kernel_width"]
+    cb = 8.0 if st["precision"] == 2 else 16.0
     return {
-        "asy_spread": 4.0 * kw * stats["asy_src"] * E * B,
-        "asy_interp": 4.0 * kw * stats["asy_tgt"] * E * B,
-        "direct": 2.0 * B * stats["pairs_dir"],
-        "asy_fft_bytes": 2.0 * 2.0 * 16.0 * stats["asy_grid_points"],
-        "channels_per_box": ch_per_box,
+        "asy_spread": ("flop", 4.0 * kw * st["spread_sc_valu"]),
+        "asy_spread_mfma": ("flop", 4.0 * kw * st["spread_sc_mfma"]),
+        "asy_interp": ("flop", 4.0 * kw * st["interp_tc_valu"]),
+        "asy_interp_mfma": ("flop", 4.0 * kw * st["interp_tc_mfma"]),
+        "direct": ("flop", 2.0 * B * st["pairs_dir"]),
+        "asy_fft": ("byte", 2.0 * 2.0 * cb * st["asy_grid_points"]),
     }
 
 
-def run_gpu(args):
+def _traffic(stage: str, tag: str):
+    """HBM bytes per launch of `stage` on workload `tag` from the committed
+    rocprofv3 counter summary, or None."""
+    path = os.path.join(HERE, "profiles", f"traffic_{stage}_{tag}.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), flags=re.M)
+    return float(m.group(1)) if m else None
+
+
+def roofline(prof: dict, works: list, mixed: bool, tag: str) -> dict:
+    """Dominant kernel over the timed region (the single-kernel stages; the
+    FFT stage is several rocFFT kernels and is reported by fft_stage);
+    works = per-plan stage_work dicts of the launches that ran (forward and
+    adjoint plans)."""
+    cand = [s for s in ("direct", "asy_spread", "asy_spread_mfma", "asy_interp",
+                        "asy_interp_mfma")
+            if prof.get(s, {}).get("launches", 0) > 0]
+    dom = max(cand, key=lambda s: prof[s]["ms"])
+    launches = prof[dom]["launches"]
+    avg_ms = prof[dom]["ms"] / launches
+    kind = works[0][dom][0]
+    per_launch = sum(w[dom][1] for w in works) / len(works)
+    if kind == "byte":
+        achieved = per_launch / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
+    else:
+        peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
+        achieved = per_launch / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = _traffic(dom, tag)
+    roof.update({"kernel": dom, "avg_launch_ms": avg_ms, "launches": launches,
+                 "algorithmic_per_launch": per_launch})
+    return roof
+
+
+def _dev_cols(a: np.ndarray, dev):
+    """(m, B) host array -> (m, B) column-major device tensor."""
+    import torch
+    return torch.as_tensor(np.asfortranarray(a).T.copy(), device=dev).t()
+
+
+# ---------------------------------------------------------------- headline
+def run_headline(args, dev):
     import torch
     import nufht_amd as na
     from nufht_amd import workloads as W
 
-    dev = torch.device("cuda:0")
     wl = W.large_uniform(batch=args.batch)
     r, k, cs, nu, tol = wl["r"], wl["k"], wl["cs"], wl["nu"], wl["tol"]
     M, N, B = r.size, k.size, cs.shape[1]
+    say(f"headline nu={nu} M={M} N={N} B={B} tol={tol} precision={args.precision}")
     t0 = time.time()
     plan = na.Plan(nu, r, k, tol=tol, precision=args.precision)
     plan_s = time.time() - t0
-    c = torch.as_tensor(np.asfortranarray(cs).T.copy(), device=dev).t()  # (M,B) column-major
-    stream = torch.cuda.current_stream()
+    c = _dev_cols(cs, dev)
 
     def step():
         g = plan.forward(c)
-        x = plan.adjoint(g)
-        return g, x
+        plan.adjoint(g)
 
     t0 = time.time()
     step()
     torch.cuda.synchronize()
-    build_s = time.time() - t0
+    first_s = time.time() - t0
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-
     na.profile(True)
     na.profile_reset()
     torch.cuda.synchronize()
@@ -98,70 +157,174 @@ def run_gpu(args):
     colpts = 2.0 * B * (M + N)
     value = colpts / ((t1 - t0) / args.steps)
 
-    # roofline of the dominant kernel (forward and adjoint launches together)
     st_f = plan.stats(B)
-    adj_stats = st_f  # adjoint work is accounted from its own plan below
     adj = na.Plan(nu, k, r, tol=tol, precision=args.precision)
     st_a = adj.stats(B)
     adj.close()
-    wf, wa = stage_work(st_f, B), stage_work(st_a, B)
-    stages = {s: v for s, v in prof.items() if v["launches"] > 0}
-    dom = max(("asy_spread", "asy_interp", "direct", "asy_fft"),
-              key=lambda s: stages.get(s, {"ms": 0})["ms"])
     mixed = st_f["precision"] == 2
-    peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
-    launches = stages[dom]["launches"]
-    avg_ms = stages[dom]["ms"] / launches
-    if dom == "asy_fft":
-        per_launch = 0.5 * (wf["asy_fft_bytes"] + wa["asy_fft_bytes"])
-        achieved = per_launch / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM_GBS, "traffic": None}
-    else:
-        per_launch = 0.5 * (wf[dom] + wa[dom])
-        achieved = per_launch / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None}
-    roof.update({"kernel": dom, "avg_launch_ms": avg_ms, "algorithmic_per_launch": per_launch})
+    prec = "mixed" if mixed else "fp64"
+    tag = f"nu5_M1e6_N1e6_B{B}_tol1e-8_{prec}"
+    works = [stage_work(st_f, B), stage_work(st_a, B)]
+    roof = roofline(prof, works, mixed, tag)
+    fft = None
+    if prof.get("asy_fft", {}).get("launches", 0) > 0:
+        f_ms = prof["asy_fft"]["ms"] / prof["asy_fft"]["launches"]
+        f_b = sum(w["asy_fft"][1] for w in works) / len(works)
+        fft = {"bound": "hbm", "avg_stage_ms": f_ms, "algorithmic_bytes": f_b,
+               "achieved_GBps": f_b / (f_ms * 1e-3) / 1e9,
+               "frac": f_b / (f_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
 
-    # plain path: one column per launch sequence (reference nufht_batch_loop pattern)
     plain = None
     if args.plain_steps > 0:
+        say("plain per-column loop")
         c1 = [c[:, t:t + 1] for t in range(B)]
         for t in range(B):
-            g1 = plan.forward(c1[t])
-            plan.adjoint(g1)
+            plan.adjoint(plan.forward(c1[t]))
         torch.cuda.synchronize()
         p0 = time.perf_counter()
         for _ in range(args.plain_steps):
             for t in range(B):
-                g1 = plan.forward(c1[t])
-                plan.adjoint(g1)
+                plan.adjoint(plan.forward(c1[t]))
         torch.cuda.synchronize()
         p1 = time.perf_counter()
-        plain_value = colpts / ((p1 - p0) / args.plain_steps)
-        plain = {"value": plain_value, "ms_per_step": (p1 - p0) * 1e3 / args.plain_steps,
-                 "speedup_batched_over_plain": value / plain_value}
+        pv = colpts / ((p1 - p0) / args.plain_steps)
+        plain = {"value": pv, "ms_per_step": (p1 - p0) * 1e3 / args.plain_steps,
+                 "speedup_batched_over_plain": value / pv}
     plan.close()
-    return {
-        "value": value, "ms_per_step": ms_step, "M": M, "N": N, "B": B, "nu": nu, "tol": tol,
-        "plan_s": plan_s, "first_call_s": build_s, "stages_ms": {s: v["ms"] / args.steps
-                                                                for s, v in stages.items()},
-        "roofline": roof, "plain": plain, "precision": "fp32-mixed" if mixed else "fp64",
-    }
+    stages = {s: round(v["ms"] / args.steps, 4) for s, v in prof.items() if v["launches"] > 0}
+    return {"value": value, "ms_per_step": ms_step, "M": M, "N": N, "B": B, "nu": nu,
+            "tol": tol, "plan_s": plan_s, "first_call_s": first_s, "stages_ms": stages,
+            "roofline": roof, "fft_stage": fft, "plain": plain, "precision": prec}
 
 
-def run_cpu_baseline(budget_s: float = 20.0):
-    """Oracle ref-algorithm CPU port on a bounded sample of the same workload:
-    the same r/k ranges with every s-th node and one coefficient column,
-    forward + adjoint, all host threads over... one column (so 1 thread)."""
+# ---------------------------------------------------------------- targets
+def run_uneven(dev, tol: float, steps: int):
+    """256 transforms of uneven size and order: MultiBatch (one launch per
+    stage, replayed as a hipGraph) for the forwards and another for the
+    adjoints (plans with r and k swapped), against the per-transform loop."""
+    import torch
+    import nufht_amd as na
+    from nufht_amd import workloads as W
+    ts = W.uneven_batch()
+    say(f"uneven batch: {len(ts)} transforms tol={tol}")
+    t0 = time.time()
+    pf = [na.Plan(t["nu"], t["r"], t["k"], tol=tol, precision="fp64") for t in ts]
+    pa = [na.Plan(t["nu"], t["k"], t["r"], tol=tol, precision="fp64") for t in ts]
+    plan_s = time.time() - t0
+    Bs = [t["cs"].shape[1] for t in ts]
+    cs = [torch.as_tensor(np.asfortranarray(t["cs"]).T.copy(), device=dev) for t in ts]
+    gs = [torch.empty((b, t["k"].size), dtype=torch.float64, device=dev) for b, t in zip(Bs, ts)]
+    xs = [torch.empty((b, t["r"].size), dtype=torch.float64, device=dev) for b, t in zip(Bs, ts)]
+    colpts = 2.0 * sum(b * (t["r"].size + t["k"].size) for b, t in zip(Bs, ts))
+    mf, ma = na.MultiBatch(pf, Bs), na.MultiBatch(pa, Bs)
+
+    def batched():
+        mf.execute(cs, gs, use_graph=True)
+        ma.execute(gs, xs, use_graph=True)
+
+    def plain():
+        for p, c in zip(pf, cs):
+            p.adjoint(p.forward(c.t()))
+
+    out = {"transforms": len(ts), "tol": tol, "plan_s": plan_s, "column_points": colpts}
+    for name, fn in (("batched", batched), ("plain", plain)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[name] = {"value": colpts / dt, "ms_per_step": dt * 1e3}
+    out["speedup_batched_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
+    mf.close()
+    ma.close()
+    for p in pf + pa:
+        p.close()
+    return out
+
+
+def run_mixed(dev, steps: int):
+    """Clustered nu = 1 workload, 16 complex = 32 real columns, tol 1e-6: the
+    FP32-mixed path against the FP64 path (fwd + adjoint each), with the
+    relative l2 distance of the mixed outputs from the FP64 outputs."""
+    import torch
+    import nufht_amd as na
+    from nufht_amd import workloads as W
+    wl = W.clustered()
+    r, k, cs = wl["r"], wl["k"], wl["cs"]
+    M, N, B = r.size, k.size, cs.shape[1]
+    say(f"mixed vs fp64: nu=1 M={M} N={N} B={B} tol=1e-6")
+    c = _dev_cols(cs, dev)
+    colpts = 2.0 * B * (M + N)
+    out, res = {"M": M, "N": N, "columns": B, "tol": 1e-6}, {}
+    for prec in ("fp64", "mixed"):
+        plan = na.Plan(1.0, r, k, tol=1e-6, precision=prec)
+        g = plan.forward(c)
+        x = plan.adjoint(g)
+        torch.cuda.synchronize()
+        res[prec] = (g.clone(), x.clone())
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.adjoint(plan.forward(c))
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[prec] = {"value": colpts / dt, "ms_per_step": dt * 1e3}
+        plan.close()
+    gf, xf = res["fp64"]
+    gm, xm = res["mixed"]
+    out["speedup_mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    out["mixed_vs_fp64_rel_l2_forward"] = float(torch.linalg.norm(gm - gf) / torch.linalg.norm(gf))
+    out["mixed_vs_fp64_rel_l2_adjoint"] = float(torch.linalg.norm(xm - xf) / torch.linalg.norm(xf))
+    return out
+
+
+def run_gradient(dev, steps: int):
+    """Gradient workload (nu = 1, M = N = 2e5, B = 8, tol 1e-10): loss
+    ||H(c; r, k) - y||^2 through the autograd layer, forward + backward
+    (dL/dc by the adjoint, dL/dr and dL/dk by the node-gradient transforms)."""
+    import torch
+    from nufht_amd import workloads as W
+    from nufht_amd.autograd import NufhtOperator
+    wl = W.gradient()
+    say("gradient: nu=1 M=N=2e5 B=8 tol=1e-10 through autograd")
+    op = NufhtOperator(1.0, tol=1e-10, precision="fp64")
+    rt = torch.tensor(wl["r"], device=dev, requires_grad=True)
+    kt = torch.tensor(wl["k"], device=dev, requires_grad=True)
+    ct = torch.tensor(wl["c0"], device=dev, requires_grad=True)
+    with torch.no_grad():
+        y = op(rt, torch.as_tensor(wl["cstar"], device=dev), kt) + \
+            torch.as_tensor(1e-3 * wl["noise"], device=dev)
+
+    def step():
+        for t in (rt, kt, ct):
+            t.grad = None
+        loss = ((op(rt, ct, kt) - y) ** 2).sum()
+        loss.backward()
+        return loss
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    op.close()
+    return {"M": wl["r"].size, "N": wl["k"].size, "columns": 8, "tol": 1e-10,
+            "ms_forward_backward": dt * 1e3}
+
+
+# ---------------------------------------------------------------- CPU baseline
+def run_cpu_baseline(sub: int = 8):
+    """The oracle's CPU restatement of the reference algorithm (1 thread, as
+    the reference runs its NUFFTs) on every sub-th node of the headline
+    grids, one column, forward + adjoint."""
     import oracle
     from nufht_amd import workloads as W
-    wl = W.large_uniform()
-    sub = 20
-    r = wl["r"][::sub]
-    k = wl["k"][::sub]
-    c = wl["cs"][::sub, :1]
+    wl = W.large_uniform(batch=1)
+    r, k, c = wl["r"][::sub], wl["k"][::sub], wl["cs"][::sub, :1]
+    say(f"cpu baseline: M=N={r.size} 1 column, 1 thread")
     t0 = time.perf_counter()
     g = oracle.nufht_batch(wl["nu"], r, c, k, tol=wl["tol"], asy_mode=1, nthreads=1)
     oracle.nufht_batch(wl["nu"], k, g, r, tol=wl["tol"], asy_mode=1, nthreads=1)
@@ -170,6 +333,19 @@ def run_cpu_baseline(budget_s: float = 20.0):
     return {"value": colpts / dt, "unit": "column-points/s", "cores": 1, "kind": "port",
             "sample": f"nu=5 tol=1e-8, every {sub}th node of the M=N=1e6 U[0,1000) grids "
                       f"(M=N={r.size}), 1 column, forward+adjoint, {dt:.1f} s"}
+
+
+def copy_calibration(dev):
+    """A plain streaming kernel over 1 GiB (b = 2 a: read 1 GiB, write 1 GiB),
+    run three times,
+    for scaling the HBM byte counters of a profiled run."""
+    import torch
+    a = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    say("copy calibration: 3 x 1 GiB device copies done")
 
 
 def main():
@@ -181,15 +357,28 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "auto", "mixed"])
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-targets", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "sharding of this transform is a design note, not a measured path.")
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build "
+              "measures one GPU's share of the transform; sharding is a design note.")
         sys.exit(2)
-    res = run_gpu(args)
+    import torch
+    dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        copy_calibration(dev)
+    res = run_headline(args, dev)
+    targets = None
+    if not args.no_targets:
+        targets = {"batched_vs_plain": res["plain"],
+                   "uneven_tol1e-10": run_uneven(dev, 1e-10, 3),
+                   "uneven_tol1e-6": run_uneven(dev, 1e-6, 3),
+                   "mixed_vs_fp64": run_mixed(dev, 3),
+                   "gradient": run_gradient(dev, 3)}
     cpu = None if args.no_cpu else run_cpu_baseline()
     out = {
-        "metric": "NUFHT column-points/s at tol 1e-8/1e-6, batched vs plain GPU; fwd+adjoint",
+        "metric": METRIC,
         "value": res["value"],
         "unit": "column-points/s",
         "n_gpus": 1,
@@ -199,19 +388,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if res["precision"] == "fp64" else "f32/f64",
+        "dtype": "f64" if res["precision"] == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": "nu=5 M=N=1e6 r,k~U[0,1000) B=16 tol=1e-8 fwd+adjoint",
                    "M": res["M"], "N": res["N"], "batch": res["B"], "nu": res["nu"],
                    "tol": res["tol"], "precision": res["precision"]},
         "roofline": res["roofline"],
+        "fft_stage": res["fft_stage"],
         "cpu_baseline": cpu,
         "plain": res["plain"],
+        "targets": targets,
         "stages_ms_per_step": res["stages_ms"],
         "plan_s": res["plan_s"],
         "first_call_s": res["first_call_s"],
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

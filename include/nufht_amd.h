@@ -99,6 +99,12 @@ typedef struct {
     int K_asy, K_loc;
     double z_split;
     long long fft_classes;                   /* distinct FFT lengths */
+    /* (source, channel) pairs spread and (target, channel) pairs interpolated
+     * by the VALU and the matrix-core kernels; a channel is one complex
+     * (exponent, column) series, each pair costs kernel_width complex
+     * multiply-adds */
+    long long spread_sc_valu, spread_sc_mfma;
+    long long interp_tc_valu, interp_tc_mfma;
 } NufhtPlanStats;
 int nufht_amd_plan_stats(const NufhtPlan *plan, int batch, NufhtPlanStats *st);
 

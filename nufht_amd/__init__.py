@@ -80,7 +80,9 @@ class PlanStats(ctypes.Structure):
         "asy_grid", "asy_grid_points", "asy_channels")] + \
         [("kernel_width", ctypes.c_int), ("precision", ctypes.c_int),
          ("K_asy", ctypes.c_int), ("K_loc", ctypes.c_int), ("z_split", ctypes.c_double),
-         ("fft_classes", ctypes.c_longlong)]
+         ("fft_classes", ctypes.c_longlong)] + \
+        [(n, ctypes.c_longlong) for n in (
+            "spread_sc_valu", "spread_sc_mfma", "interp_tc_valu", "interp_tc_mfma")]
 
 
 # Every exported symbol of the C ABI, as declared in include/*.h.
@@ -385,8 +387,9 @@ class MultiBatch:
 
 # ---------------------------------------------------------------------------
 # per-stage device timing (HIP events around each stage launch)
-STAGES = ["zero", "permute", "direct", "loc_project", "loc_eval", "asy_spread", "asy_fft",
-          "asy_interp", "grad_combine"]
+STAGES = ["zero", "permute", "direct", "loc_project", "loc_eval", "asy_grid_zero",
+          "asy_spread", "asy_spread_mfma", "asy_fft", "asy_interp", "asy_interp_mfma",
+          "grad_combine"]
 
 
 def profile(enable: bool = True):

@@ -61,11 +61,13 @@ extern "C" const char *nufht_amd_version(void) { return "nufht-amd 0.1 gfx950"; 
 
 // ---------------------------------------------------------------------------
 // per-stage timing
-enum Stage { ST_ZERO, ST_PERM, ST_DIR, ST_LOCP, ST_LOCE, ST_SPREAD, ST_FFT, ST_INTERP,
-             ST_GRAD, ST_COUNT };
+// (one kernel per stage, except asy_fft: the rocFFT launches of all classes)
+enum Stage { ST_ZERO, ST_PERM, ST_DIR, ST_LOCP, ST_LOCE, ST_GZERO, ST_SPREAD, ST_SPREAD_MF,
+             ST_FFT, ST_INTERP, ST_INTERP_MF, ST_GRAD, ST_COUNT };
 static const char *kStageNames[ST_COUNT] = {"zero", "permute", "direct", "loc_project",
-                                            "loc_eval", "asy_spread", "asy_fft",
-                                            "asy_interp", "grad_combine"};
+                                            "loc_eval", "asy_grid_zero", "asy_spread",
+                                            "asy_spread_mfma", "asy_fft", "asy_interp",
+                                            "asy_interp_mfma", "grad_combine"};
 struct PendingEv { hipEvent_t a, b; int stage; };
 static std::mutex g_prof_mu;
 static bool g_prof_on = false;
@@ -588,6 +590,8 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         std::vector<AsyItem> &items = dense ? P->adense : P->aspread;
         std::vector<int2> &ranges = dense ? P->adrange : P->arange;
         for (int c0 = 0; c0 < T.B; c0 += tb) {
+            long long chans = (long long)A.E * std::min(tb, T.B - c0);
+            (dense ? P->stats.spread_sc_mfma : P->stats.spread_sc_valu) += (long long)A.nj * chans;
             for (int s = 0; s < A.Nf; s += step) {
                 int np = std::min(step, A.Nf - s);
                 int lmin = INT32_MAX, lmax = INT32_MIN;
@@ -619,8 +623,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
             // merged into VALU items of up to 256/tb targets
             int run0 = -1;
             auto flush = [&](int upto) {
-                for (int s = run0; run0 >= 0 && s < upto; s += tp)
+                for (int s = run0; run0 >= 0 && s < upto; s += tp) {
                     P->ainterp.push_back({box, s, std::min(tp, upto - s), c0, tb, 0});
+                    P->stats.interp_tc_valu += (long long)std::min(tp, upto - s) * chans;
+                }
                 run0 = -1;
             };
             for (int s = 0; s < A.ni; s += 16) {
@@ -631,6 +637,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 if (m0l - m0f + T.kw + 2 <= 16) {
                     flush(s);
                     P->aimf.push_back({box, s, cnt, c0, tb, m0f - 1});
+                    P->stats.interp_tc_mfma += (long long)cnt * chans;
                 } else if (run0 < 0) {
                     run0 = s;
                 }
@@ -719,19 +726,22 @@ template <typename R, int EM>
 static void launch_asy(Program *P, hipStream_t st)
 {
     typedef typename Cplx<R>::T G;
-    {
+    if (!P->azero.empty()) {
+        StageTimer tm(ST_GZERO, st);
+        hipLaunchKernelGGL((k_grid_zero<R>), dim3((unsigned)P->azero.size()), dim3(256), 0, st,
+                           P->d_tfs, P->d_abox, P->d_azero, (G *)P->d_grid);
+    }
+    if (!P->aspread.empty()) {
         StageTimer tm(ST_SPREAD, st);
-        if (!P->azero.empty())
-            hipLaunchKernelGGL((k_grid_zero<R>), dim3((unsigned)P->azero.size()), dim3(256), 0, st,
-                               P->d_tfs, P->d_abox, P->d_azero, (G *)P->d_grid);
-        if (!P->aspread.empty())
-            hipLaunchKernelGGL((k_asy_spread<R, EM>), dim3((unsigned)P->aspread.size()), dim3(256),
-                               0, st, P->d_tfs, P->d_abox, P->d_aspread, P->d_arange,
-                               (G *)P->d_grid, P->d_ft);
-        if (!P->adense.empty())
-            hipLaunchKernelGGL((k_asy_spread_mfma<R, EM>), dim3((unsigned)P->adense.size()),
-                               dim3(64), 0, st, P->d_tfs, P->d_abox, P->d_adense, P->d_adrange,
-                               (G *)P->d_grid, P->d_ft);
+        hipLaunchKernelGGL((k_asy_spread<R, EM>), dim3((unsigned)P->aspread.size()), dim3(256),
+                           0, st, P->d_tfs, P->d_abox, P->d_aspread, P->d_arange,
+                           (G *)P->d_grid, P->d_ft);
+    }
+    if (!P->adense.empty()) {
+        StageTimer tm(ST_SPREAD_MF, st);
+        hipLaunchKernelGGL((k_asy_spread_mfma<R, EM>), dim3((unsigned)P->adense.size()),
+                           dim3(64), 0, st, P->d_tfs, P->d_abox, P->d_adense, P->d_adrange,
+                           (G *)P->d_grid, P->d_ft);
     }
     {
         StageTimer tm(ST_FFT, st);
@@ -741,17 +751,18 @@ static void launch_asy(Program *P, hipStream_t st)
             rocfft_execute(c.plan, &buf, nullptr, c.info);
         }
     }
-    {
+    if (!P->ainterp.empty()) {
         StageTimer tm(ST_INTERP, st);
         int tmx = P->interp_tmax;
         size_t lds = interp_lds_off(tmx, 2, sizeof(R)) * sizeof(double);
-        if (!P->ainterp.empty())
-            hipLaunchKernelGGL((k_asy_interp<R, EM>), dim3((unsigned)P->ainterp.size()), dim3(256),
-                               lds, st, P->d_tfs, P->d_abox, P->d_ainterp, (const G *)P->d_grid,
-                               P->d_ft, tmx);
-        if (!P->aimf.empty())
-            hipLaunchKernelGGL((k_asy_interp_mfma<R, EM>), dim3((unsigned)P->aimf.size()), dim3(64),
-                               0, st, P->d_tfs, P->d_abox, P->d_aimf, (const G *)P->d_grid, P->d_ft);
+        hipLaunchKernelGGL((k_asy_interp<R, EM>), dim3((unsigned)P->ainterp.size()), dim3(256),
+                           lds, st, P->d_tfs, P->d_abox, P->d_ainterp, (const G *)P->d_grid,
+                           P->d_ft, tmx);
+    }
+    if (!P->aimf.empty()) {
+        StageTimer tm(ST_INTERP_MF, st);
+        hipLaunchKernelGGL((k_asy_interp_mfma<R, EM>), dim3((unsigned)P->aimf.size()), dim3(64),
+                           0, st, P->d_tfs, P->d_abox, P->d_aimf, (const G *)P->d_grid, P->d_ft);
     }
 }
 

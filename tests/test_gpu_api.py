@@ -108,19 +108,25 @@ def test_node_gradients_against_finite_differences(device, nu):
     gradient; the exact analytic gradient (FP64 direct sums) is compared per
     coordinate."""
     import torch
-    import nufht_amd as na
+    from nufht_amd.autograd import NufhtOperator
     wl = W.gradient(m=20000, n=20000)
     r, k, c, cstar = wl["r"], wl["k"], wl["c0"], wl["cstar"]
-    p = na.Plan(nu, r, k, tol=1e-10)
-    say(f"gradient nu={nu} M=N=2e4 B=8")
-    Hs = p.forward(torch.as_tensor(cstar, device=device)).cpu().numpy()
-    y = Hs + 1e-3 * wl["noise"]
-    H = p.forward(torch.as_tensor(c, device=device)).cpu().numpy()
-    R = H - y
-    rho = torch.as_tensor(2 * R, device=device)
-    gr, gk = p.grad_nodes(torch.as_tensor(c, device=device), rho)
-    dc = p.adjoint(rho).cpu().numpy()
-    gr, gk = gr.cpu().numpy(), gk.cpu().numpy()
+    op = NufhtOperator(nu, tol=1e-10, precision="fp64")
+    rt = torch.tensor(r, device=device, requires_grad=True)
+    kt = torch.tensor(k, device=device, requires_grad=True)
+    ct = torch.tensor(c, device=device, requires_grad=True)
+    say(f"gradient nu={nu} M=N=2e4 B=8 through autograd")
+    with torch.no_grad():
+        y = op(rt, torch.as_tensor(cstar, device=device), kt) + \
+            torch.as_tensor(1e-3 * wl["noise"], device=device)
+    H = op(rt, ct, kt)
+    loss = ((H - y) ** 2).sum()
+    loss.backward()
+    R = (H - y).detach().cpu().numpy()
+    gr, gk, dc = rt.grad.cpu().numpy(), kt.grad.cpu().numpy(), ct.grad.cpu().numpy()
+    # the autograd layer and the plan's own entry point agree
+    pr, pk = op._plan.grad_nodes(ct.detach(), torch.as_tensor(2 * R, device=device))
+    assert rel_l2(pr.cpu().numpy(), gr) < 1e-13 and rel_l2(pk.cpu().numpy(), gk) < 1e-13
     s = W.Stream(52, 5)
     js = np.unique((s.uniform(50) * r.size).astype(int))
     fd = np.array([_loss_fd_r(nu, r, k, c, R, j, 1e-6) for j in js])
@@ -144,7 +150,7 @@ def test_node_gradients_against_finite_differences(device, nu):
     fdk, exk = np.array(fdk), np.array(exk)
     assert np.max(np.abs(gk[iks] - fdk)) < 1e-6 * np.max(np.abs(fdk))
     assert np.max(np.abs(gk[iks] - exk) / (np.abs(exk) + 1e-6 * np.max(np.abs(exk)))) < 1e-8
-    p.close()
+    op.close()
 
 
 def test_uneven_multi_batch_and_graph_replays(device):
