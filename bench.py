@@ -55,18 +55,19 @@ def say(msg: str):
 def stage_work(st: dict, B: int) -> dict:
     """Algorithmic work of one launch of each single-kernel stage.
 
-    A channel is one complex series (exponent e, column t).  Spreading a
-    source into one channel is kernel_width complex multiply-adds of a real
-    kernel value (4 flops each); interpolating a target from one channel is
-    the same.  Direct sums: 2 B flops per (target, source) pair, the Bessel
-    evaluation not counted.  FFT: one read and one write of every complex
-    grid point of FP64 (16 B) or FP32 (8 B) per pass, two passes counted.
+    A channel is one complex series (exponent e, column t); a box has
+    C = 2 (K_asy + 1) B of them.  The per-unit figures are the survey's
+    cost model: 2 w flops per (source, channel) spread, 4 w flops per
+    (target, channel) interpolated (w = kernel width), 2 B flops per direct
+    (target, source) pair with the Bessel evaluation not counted.  FFT: one
+    read and one write of every complex grid point of FP64 (16 B) or FP32
+    (8 B) per pass, two passes counted.
     """
     kw = st["kernel_width"]
     cb = 8.0 if st["precision"] == 2 else 16.0
     return {
-        "asy_spread": ("flop", 4.0 * kw * st["spread_sc_valu"]),
-        "asy_spread_mfma": ("flop", 4.0 * kw * st["spread_sc_mfma"]),
+        "asy_spread": ("flop", 2.0 * kw * st["spread_sc_valu"]),
+        "asy_spread_mfma": ("flop", 2.0 * kw * st["spread_sc_mfma"]),
         "asy_interp": ("flop", 4.0 * kw * st["interp_tc_valu"]),
         "asy_interp_mfma": ("flop", 4.0 * kw * st["interp_tc_mfma"]),
         "direct": ("flop", 2.0 * B * st["pairs_dir"]),
