@@ -23,6 +23,9 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17", "-munsafe-fp-atomics",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # matrix-core accumulators in ordinary VGPRs: without it the
+             # compiler copies them through AGPRs around every MFMA of a loop
+             "-mllvm", "-amdgpu-mfma-vgpr-form",
              "-Wno-unused-result", "-Wno-unused-value"]
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wno-unused-function"]
 

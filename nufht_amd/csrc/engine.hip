@@ -216,11 +216,25 @@ static const KernelTable &kernel_table(int kw, double beta)
         }
         F[q] = s;
     }
-    t->coef.assign(kNFT, 0.0);
+    t->coef.assign(kNFT + kTapMax * kEsN, 0.0);
     for (int n = 0; n < kNFT; ++n) {
         double s = 0.0;
         for (int q = 0; q < Q; ++q) s += F[q] * std::cos(n * M_PI * (q + 0.5) / Q);
         t->coef[n] = s * (n == 0 ? 1.0 : 2.0) / Q;
+    }
+    // kernel values at the taps as Chebyshev series in the cell offset
+    for (int k = 0; k < kTapMax; ++k) {
+        double fv[kEsN];
+        for (int j = 0; j < kEsN; ++j) {
+            double y = std::cos(M_PI * (j + 0.5) / kEsN), f = 0.5 * (y + 1.0);
+            double u = (2.0 / kw) * (k - 0.5 * kw + f), tt = 1.0 - u * u;
+            fv[j] = tt > 0 ? std::exp(beta * (std::sqrt(tt) - 1.0)) : 0.0;
+        }
+        for (int n = 0; n < kEsN; ++n) {
+            double s = 0.0;
+            for (int j = 0; j < kEsN; ++j) s += fv[j] * std::cos(n * M_PI * (j + 0.5) / kEsN);
+            t->coef[kNFT + k * kEsN + n] = s * (n == 0 ? 1.0 : 2.0) / kEsN;
+        }
     }
     cache.push_back(t);
     return *t;
@@ -234,6 +248,11 @@ static const int kSpreadSplit = 2048;
 static const int kDenseSplit = 4096;
 // sources per grid cell from which a box uses the split-K spread kernel
 static const double kDenseCell = 32.0;
+// (NUFHT_AMD_DENSE_CELL overrides the threshold, for tuning)
+static double g_dense_cell = [] {
+    const char *e = getenv("NUFHT_AMD_DENSE_CELL");
+    return e ? atof(e) : kDenseCell;
+}();
 
 // FFT lengths restricted to 2^a and 3*2^a (prebuilt rocFFT kernels)
 static int fft_length(long long need)
@@ -282,10 +301,12 @@ struct Program {
     std::vector<AsyBoxD> abox;
     AsyBoxD *d_abox = nullptr;
     std::vector<AsyItem> aspread, ainterp, azero, adense, aimf;
-    std::vector<int2> arange, adrange;
+    std::vector<int2> arange;
+    std::vector<SpreadRange> adrange;
     AsyItem *d_aspread = nullptr, *d_ainterp = nullptr, *d_azero = nullptr, *d_adense = nullptr;
     AsyItem *d_aimf = nullptr;
-    int2 *d_arange = nullptr, *d_adrange = nullptr;
+    int2 *d_arange = nullptr;
+    SpreadRange *d_adrange = nullptr;
     int interp_tmax = 16;
     void *d_grid = nullptr;
     long long grid_elems = 0;
@@ -584,37 +605,61 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         int sp = 256 / std::max(1, tb / 2);            // spread positions per item
         const double *rb = h->r + A.j0, *re = h->r + A.j0 + A.nj;
         // source-dense boxes (many sources per grid cell) use the split-K kernel
-        bool dense = (double)A.nj >= kDenseCell * (2.0 * A.L + 1.0);
-        int step = dense ? kDenseP : sp;
-        int split = dense ? kDenseSplit : kSpreadSplit;
-        std::vector<AsyItem> &items = dense ? P->adense : P->aspread;
-        std::vector<int2> &ranges = dense ? P->adrange : P->arange;
+        bool dense = (double)A.nj >= g_dense_cell * (2.0 * A.L + 1.0);
+        // sources whose kernel support reaches grid positions [s, s + np)
+        auto src_range = [&](int s, int np) {
+            int lmin = INT32_MAX, lmax = INT32_MIN;
+            for (int p = s; p < s + np; ++p) {
+                int l = p <= A.L ? p : p - A.Nf;
+                if (l < -A.L || l > A.L) continue;
+                lmin = std::min(lmin, l);
+                lmax = std::max(lmax, l);
+            }
+            int2 jr = make_int2(0, 0);
+            if (lmin <= lmax) {
+                double xlo = A.C + (lmin - 1) * A.h - A.a1, xhi = A.C + (lmax + 1) * A.h + A.a1;
+                jr.x = A.j0 + (int)(std::lower_bound(rb, re, xlo) - rb);
+                jr.y = A.j0 + (int)(std::upper_bound(rb, re, xhi) - rb);
+            }
+            return jr;
+        };
         for (int c0 = 0; c0 < T.B; c0 += tb) {
             long long chans = (long long)A.E * std::min(tb, T.B - c0);
             (dense ? P->stats.spread_sc_mfma : P->stats.spread_sc_valu) += (long long)A.nj * chans;
-            for (int s = 0; s < A.Nf; s += step) {
-                int np = std::min(step, A.Nf - s);
-                int lmin = INT32_MAX, lmax = INT32_MIN;
-                for (int p = s; p < s + np; ++p) {
-                    int l = p <= A.L ? p : p - A.Nf;
-                    if (l < -A.L || l > A.L) continue;
-                    lmin = std::min(lmin, l);
-                    lmax = std::max(lmax, l);
+            if (dense) {
+                for (int s = 0; s < A.Nf; s += kSP) {
+                    int np = std::min(kSP, A.Nf - s);
+                    SpreadRange sr;
+                    sr.piece = src_range(s, np);
+                    for (int t = 0; t < 4; ++t)
+                        sr.tile[t] = 4 * t < np ? src_range(s + 4 * t, std::min(4, np - 4 * t))
+                                                : make_int2(0, 0);
+                    int jlo = sr.piece.x, jhi = sr.piece.y;
+                    if (jhi - jlo <= kDenseSplit) {
+                        P->adense.push_back({box, s, np, c0, tb, 0});
+                        P->adrange.push_back(sr);
+                    } else {
+                        P->azero.push_back({box, s, np, c0, tb, 0});
+                        for (int j = jlo; j < jhi; j += kDenseSplit) {
+                            sr.piece = make_int2(j, std::min(jhi, j + kDenseSplit));
+                            P->adense.push_back({box, s, np, c0, tb, 1});
+                            P->adrange.push_back(sr);
+                        }
+                    }
                 }
-                int jlo = 0, jhi = 0;
-                if (lmin <= lmax) {
-                    double xlo = A.C + (lmin - 1) * A.h - A.a1, xhi = A.C + (lmax + 1) * A.h + A.a1;
-                    jlo = A.j0 + (int)(std::lower_bound(rb, re, xlo) - rb);
-                    jhi = A.j0 + (int)(std::upper_bound(rb, re, xhi) - rb);
-                }
-                if (jhi - jlo <= split) {
-                    items.push_back({box, s, np, c0, tb, 0});
-                    ranges.push_back(make_int2(jlo, jhi));
-                } else {
-                    P->azero.push_back({box, s, np, c0, tb, 0});
-                    for (int j = jlo; j < jhi; j += split) {
-                        items.push_back({box, s, np, c0, tb, 1});
-                        ranges.push_back(make_int2(j, std::min(jhi, j + split)));
+            } else {
+                for (int s = 0; s < A.Nf; s += sp) {
+                    int np = std::min(sp, A.Nf - s);
+                    int2 jr = src_range(s, np);
+                    if (jr.y - jr.x <= kSpreadSplit) {
+                        P->aspread.push_back({box, s, np, c0, tb, 0});
+                        P->arange.push_back(jr);
+                    } else {
+                        P->azero.push_back({box, s, np, c0, tb, 0});
+                        for (int j = jr.x; j < jr.y; j += kSpreadSplit) {
+                            P->aspread.push_back({box, s, np, c0, tb, 1});
+                            P->arange.push_back(make_int2(j, std::min(jr.y, j + kSpreadSplit)));
+                        }
                     }
                 }
             }
@@ -740,7 +785,7 @@ static void launch_asy(Program *P, hipStream_t st)
     if (!P->adense.empty()) {
         StageTimer tm(ST_SPREAD_MF, st);
         hipLaunchKernelGGL((k_asy_spread_mfma<R, EM>), dim3((unsigned)P->adense.size()),
-                           dim3(64), 0, st, P->d_tfs, P->d_abox, P->d_adense, P->d_adrange,
+                           dim3(256), 0, st, P->d_tfs, P->d_abox, P->d_adense, P->d_adrange,
                            (G *)P->d_grid, P->d_ft);
     }
     {

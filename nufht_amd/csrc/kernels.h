@@ -69,6 +69,73 @@ __device__ __forceinline__ double ft_eval(const double *c, double k, double kmax
     return fma(y, b1, c[0] - b2);
 }
 
+// sin and cos of v: v = n pi/2 + r by Cody-Waite reduction (pi/2 split in
+// three parts, the first two with 30 significant bits so that n P1 and n P2
+// are exact for |n| < 2^22), Taylor polynomials of sin and cos on
+// |r| <= pi/4 (coefficients (-1)^k / j!, truncation below 1e-19), and the
+// quadrant of n.  Arguments beyond 2^21 go to the library routine.
+__device__ __forceinline__ void sincos_cw(double v, double *s, double *c)
+{
+    if (!(fabs(v) < 2097152.0)) {
+        sincos(v, s, c);
+        return;
+    }
+    const double P1 = 0x1.921fb54p+0, P2 = 0x1.10b46118p-30, P3 = 0x1.313198a2e0370p-61;
+    double n = rint(v * 0x1.45f306dc9c883p-1);
+    double r = fma(-n, P1, v);
+    r = fma(-n, P2, r);
+    r = fma(-n, P3, r);
+    double r2 = r * r;
+    double ps = 0x1.952c77030ad4ap-49;
+    ps = fma(ps, r2, -0x1.ae7f3e733b81fp-41);
+    ps = fma(ps, r2, 0x1.6124613a86d09p-33);
+    ps = fma(ps, r2, -0x1.ae64567f544e4p-26);
+    ps = fma(ps, r2, 0x1.71de3a556c734p-19);
+    ps = fma(ps, r2, -0x1.a01a01a01a01ap-13);
+    ps = fma(ps, r2, 0x1.1111111111111p-7);
+    ps = fma(ps, r2, -0x1.5555555555555p-3);
+    double sr = fma(ps * r2, r, r);
+    double pc = -0x1.6827863b97d97p-53;
+    pc = fma(pc, r2, 0x1.ae7f3e733b81fp-45);
+    pc = fma(pc, r2, -0x1.93974a8c07c9dp-37);
+    pc = fma(pc, r2, 0x1.1eed8eff8d898p-29);
+    pc = fma(pc, r2, -0x1.27e4fb7789f5cp-22);
+    pc = fma(pc, r2, 0x1.a01a01a01a01ap-16);
+    pc = fma(pc, r2, -0x1.6c16c16c16c17p-10);
+    pc = fma(pc, r2, 0x1.5555555555555p-5);
+    pc = fma(pc, r2, -0.5);
+    double cr = fma(pc, r2, 1.0);
+    int q = (int)((long long)n & 3);
+    double ss = (q & 1) ? cr : sr, cc = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -ss : ss;
+    *c = ((q + 1) & 2) ? -cc : cc;
+}
+
+// Kernel values at the kw + 1 grid points l = lf + k of a point x, with
+// lf = ceil((x - a1) / h): psi_k(f) = ES((2 / kw) (k - kw / 2 + f)) for the
+// cell offset f = (lf h - x) / h + kw / 2 in [0, 1), each tap a Chebyshev
+// series of degree kEsDeg in y = 2 f - 1 (coefficients per (kw, beta) in the
+// kernel table, after the kNFT transform coefficients).
+constexpr int kEsDeg = 14;
+constexpr int kEsN = kEsDeg + 1;
+template <typename R>
+__device__ __forceinline__ void cheb_basis(R y, R (&t)[kEsN])
+{
+    t[0] = (R)1;
+    t[1] = y;
+    const R y2 = y + y;
+#pragma unroll
+    for (int n = 2; n < kEsN; ++n) t[n] = fma(y2, t[n - 1], -t[n - 2]);
+}
+template <typename R>
+__device__ __forceinline__ R es_tap(const double *__restrict__ c, const R (&t)[kEsN])
+{
+    R s = (R)0;
+#pragma unroll
+    for (int n = kEsN - 1; n >= 0; --n) s = fma((R)c[n], t[n], s);
+    return s;
+}
+
 __device__ __forceinline__ double es_kernel(double u, double beta)
 {
     double t = 1.0 - u * u;
@@ -525,9 +592,6 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
     }
 }
 
-// Grid positions per tile of a source-dense (MFMA) spread item.
-constexpr int kDenseP = 4;
-
 // ------------------------------------------------------------------------
 // MFMA formulations of the asymptotic stage for dense boxes.
 //
@@ -556,143 +620,164 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-// Spread, source-dense boxes.  For an item of 4 grid positions, columns
-// [col0, col0 + tb) and a source range, with nsub = 16 / tb:
-//   G[p][e][t] = sum_q psi(l_p h - x_q) b_q (rho/r_q)^e c[q][t],  b = e^{iDx} r^{-1/2}
-// as matrix products over q: rows (p, e1) with e1 < 4, columns (t, e2)
-// with e2 < nsub, e = (g*4 + e1)*nsub + e2 for exponent group g; A carries
-// psi b rho^{(4g+e1) nsub}, B carries c rho^{e2}.  Re and Im are separate
-// products sharing B.
-constexpr int kMfQ = 32;          // sources staged per LDS pass
+// Spread, source-dense boxes, on the matrix cores.  A block of 4 waves
+// covers kSP = 16 consecutive grid positions (wave w: positions 4w..4w+3 of
+// the item), columns [col0, col0 + tb) and a source range (the union of the
+// waves' ranges, or one piece of it when split); with nsub = 16 / tb:
+//   G[p][e][t] = sum_q psi(l_p h - x_q) u_q^(e) c[q][t],  u^(e) = e^{iDx} r^{-1/2} (rho/r)^e
+// as matrix products over q: rows (p, e1) with e1 < 4, columns (t, e2) with
+// e2 < nsub, e = (g*4 + e1)*nsub + e2 for exponent group g; A carries
+// psi u^((4g+e1) nsub), B carries c (rho/r)^{e2}.  Re and Im are separate
+// products sharing B.  Per source, the phase, powers, kernel taps and
+// coefficients are computed once per block and shared through LDS.
+constexpr int kSQ = 64;           // sources staged per LDS pass
+constexpr int kSP = 16;           // grid positions per block
 
-template <typename R, int EMAX>
-__global__ __launch_bounds__(64) void k_asy_spread_mfma(const TfDesc *__restrict__ tfs,
-                                                        const AsyBoxD *__restrict__ boxes,
-                                                        const AsyItem *__restrict__ items,
-                                                        const int2 *__restrict__ ranges,
-                                                        typename Cplx<R>::T *__restrict__ grid,
-                                                        const double *__restrict__ ft)
+struct SpreadRange { int2 piece; int2 tile[4]; };
+
+template <typename R, int EMAX> struct SpreadLds {
+    typedef typename Cplx<R>::T C2;
+    R psi[kSQ][kSP];              // kernel value of source q at the block's positions
+    C2 u[kSQ][EMAX];              // e^{iDx} r^{-1/2} (rho/r)^e
+    R bop[kSQ][16];               // B operand: c[q][t] (rho/r)^{e2}, column j = (t, e2)
+};
+
+// Products of one staged chunk for NG exponent groups.  Sources outside a
+// wave's range have zero kernel values at its positions, and the staged
+// arrays are zero past the chunk's end, so the loop needs no masks: the
+// accumulators stay in registers across steps.
+template <typename R, int EMAX, int NG>
+__device__ __forceinline__ void spread_chunk(const SpreadLds<R, EMAX> &L, int qlo, int qhi,
+                                             int prow, int E, int nsub, int ka, int e1a, int jb,
+                                             typename Mfma<R>::V (&acc_r)[NG],
+                                             typename Mfma<R>::V (&acc_i)[NG])
+{
+    typedef typename Cplx<R>::T C2;
+    int eidx[NG];
+    bool eok[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        int ea = (4 * g + e1a) * nsub;
+        eok[g] = ea < E;
+        eidx[g] = ea < E ? ea : 0;
+    }
+    for (int s = qlo & ~3; s < qhi; s += 4) {
+        const int q = s + ka;
+        const R psi = L.psi[q][prow];
+        const R bval = L.bop[q][jb];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            C2 u = L.u[q][eidx[g]];
+            R m = eok[g] ? psi : (R)0;
+            acc_r[g] = Mfma<R>::mma(m * u.x, bval, acc_r[g]);
+            acc_i[g] = Mfma<R>::mma(m * u.y, bval, acc_i[g]);
+        }
+    }
+}
+
+// Spread, source-dense boxes, on the matrix cores.  A block of 4 waves
+// covers kSP = 16 consecutive grid positions (wave w: positions 4w..4w+3 of
+// the item), columns [col0, col0 + tb) and a source range (the union of the
+// waves' ranges, or one piece of it when split); with nsub = 16 / tb:
+//   G[p][e][t] = sum_q psi(l_p h - x_q) u_q^(e) c[q][t],  u^(e) = e^{iDx} r^{-1/2} (rho/r)^e
+// as matrix products over q: rows (p, e1) with e1 < 4, columns (t, e2) with
+// e2 < nsub, e = (g*4 + e1)*nsub + e2 for exponent group g; A carries
+// psi u^((4g+e1) nsub), B carries c (rho/r)^{e2}.  Re and Im are separate
+// products sharing B.  Per source, the phase, powers, kernel values and the
+// B operand are computed once per block and shared through LDS.
+template <typename R, int EMAX, int NG>
+__device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD &bx,
+                                                 const AsyItem &it, int2 piece, int2 trg,
+                                                 typename Cplx<R>::T *__restrict__ grid,
+                                                 const double *__restrict__ ft,
+                                                 SpreadLds<R, EMAX> &L)
 {
     typedef typename Cplx<R>::T C2;
     typedef typename Mfma<R>::V V;
-    constexpr int GMAX = (EMAX + 3) / 4;
-    constexpr int RPN = (EMAX > 16 ? EMAX : 16) + 1;   // powers used by A (< E) and B (< nsub)
-    constexpr int CPL = kMfQ * 16 / 64;                 // coefficients staged per lane
-    __shared__ R s_psi[kMfQ][4];                        // kernel taps at the item's positions
-    __shared__ C2 s_b[kMfQ];                            // e^{iDx} r^{-1/2}
-    __shared__ R s_c[kMfQ][16];
-    __shared__ R s_rp[kMfQ][RPN];
-    AsyItem it = items[blockIdx.x];
-    int2 jr = ranges[blockIdx.x];
-    AsyBoxD bx = boxes[it.box];
-    const TfDesc &T = tfs[bx.tf];
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int tb = it.tb, nsub = 16 / tb;
     const int E = bx.E, kw = T.kw;
-    const int G = (E + 4 * nsub - 1) / (4 * nsub);
-    const int npw = max(E, nsub);
-    const double inv_a1 = 1.0 / bx.a1;
-    const int ka = lane >> 4;
-    const int ia = lane & 15;
+    const double inv_h = 1.0 / bx.h;
+    const int ka = lane >> 4, ia = lane & 15;
     const int pa = ia >> 2, e1a = ia & 3;
     const int jb = lane & 15;
-    const int tbk = jb % tb, e2b = jb / tb;
-    // the item's grid positions (l values; out-of-range ones get no taps)
-    int lpos[4];
+    // the block's first position as an l value; positions past the item or
+    // outside [-L, L] get no kernel values
+    const int qa = max(trg.x, piece.x), qb = min(trg.y, piece.y);
+    V acc_r[NG], acc_i[NG];
 #pragma unroll
-    for (int pp = 0; pp < 4; ++pp) {
-        int pq = it.start + pp;
-        int lq = pq <= bx.L ? pq : pq - bx.Nf;
-        lpos[pp] = (pp < it.count && lq >= -bx.L && lq <= bx.L) ? lq : INT_MIN / 2;
-    }
-    V acc_r[GMAX], acc_i[GMAX];
-#pragma unroll
-    for (int g = 0; g < GMAX; ++g)
+    for (int g = 0; g < NG; ++g)
 #pragma unroll
         for (int v = 0; v < 4; ++v) { acc_r[g][v] = (R)0; acc_i[g][v] = (R)0; }
-    // register prefetch of one chunk: lane q % kMfQ loads r; every lane CPL coefficients
-    const int qs = lane & (kMfQ - 1);
-    double r_pre;
-    R c_pre[CPL];
-    auto prefetch = [&](int c0) {
-        int cn = min(kMfQ, jr.y - c0);
-        r_pre = qs < cn ? T.r[c0 + qs] : bx.C;
-#pragma unroll
-        for (int u = 0; u < CPL; ++u) {
-            int q2 = lane + 64 * u, q = q2 >> 4, tt = q2 & 15;
-            int tc = it.col0 + tt;
-            c_pre[u] = (q < cn && tt < tb && tc < T.B) ? (R)T.cs[(long long)tc * T.ld_cs + c0 + q] : (R)0;
-        }
-    };
-    if (jr.x < jr.y) prefetch(jr.x);
-    for (int c0 = jr.x; c0 < jr.y; c0 += kMfQ) {
-        int cn = min(kMfQ, jr.y - c0);
-        // stage the prefetched chunk: lanes < kMfQ take phases and powers,
-        // lanes >= kMfQ the kernel taps (same source q = lane % kMfQ)
-        {
-            int q = qs;
-            bool live = q < cn;
-            double rj = r_pre;
+    for (int c0 = piece.x; c0 < piece.y; c0 += kSQ) {
+        const int cn = min(kSQ, piece.y - c0);
+        // ---- staging, one source per lane.  Wave 0: phases and powers;
+        // wave 1: kernel values at the block's positions; waves 2-3: B operand.
+        if (wv == 0) {
+            const int q = lane;
+            double rj = q < cn ? T.r[c0 + q] : bx.C;
             double x = rj - bx.C;
-            if (lane < kMfQ) {
-                double sn, cs;
-                sincos(bx.D * x, &sn, &cs);
-                double rs = live ? 1.0 / sqrt(rj) : 0.0;
-                C2 b;
-                b.x = (R)(cs * rs);
-                b.y = (R)(sn * rs);
-                s_b[q] = b;
-                double ri = live ? bx.rho / rj : 0.0, rp = 1.0;
-                for (int e = 0; e < npw; ++e) {
-                    s_rp[q][e] = (R)rp;
-                    rp *= ri;
-                }
-            } else {
-                int lf = (int)ceil((x - bx.a1) / bx.h);
+            double sn, cs;
+            sincos_cw(bx.D * x, &sn, &cs);
+            double rs = q < cn ? 1.0 / sqrt(rj) : 0.0;
+            double br = cs * rs, bi = sn * rs;
+            double ri = q < cn ? bx.rho / rj : 0.0, rp = 1.0;
+            for (int e = 0; e < E; ++e) {
+                C2 u;
+                u.x = (R)(br * rp);
+                u.y = (R)(bi * rp);
+                L.u[q][e] = u;
+                rp *= ri;
+            }
+        } else if (wv == 1) {
+            const int q = lane;
 #pragma unroll
-                for (int pp = 0; pp < 4; ++pp) {
-                    int k = lpos[pp] - lf;
-                    R psi = (R)0;
-                    if (live && k >= 0 && k <= kw)
-                        psi = es_eval((R)(((double)lpos[pp] * bx.h - x) * inv_a1), (R)T.beta);
-                    s_psi[q][pp] = psi;
+            for (int pi = 0; pi < kSP; ++pi) L.psi[q][pi] = (R)0;
+            if (q < cn) {
+                double x = T.r[c0 + q] - bx.C;
+                int lf = (int)ceil((x - bx.a1) * inv_h);
+                double f = ((double)lf * bx.h - x) * inv_h + 0.5 * kw;
+                R tb_[kEsN];
+                cheb_basis((R)(2.0 * f - 1.0), tb_);
+                const double *tc = ft + T.ft_off + kNFT;
+                for (int k = 0; k <= kw; ++k) {
+                    int l = lf + k;
+                    int pos = l >= 0 ? l : l + bx.Nf;
+                    int pi = pos - it.start;
+                    if (pi >= 0 && pi < it.count && l >= -bx.L && l <= bx.L)
+                        L.psi[q][pi] = es_tap(tc + k * kEsN, tb_);
                 }
             }
-#pragma unroll
-            for (int u = 0; u < CPL; ++u) {
-                int q2 = lane + 64 * u;
-                s_c[q2 >> 4][q2 & 15] = c_pre[u];
+        } else {
+            for (int idx = tid - 128; idx < kSQ * 16; idx += 128) {
+                int q = idx & (kSQ - 1), j = idx >> 6;
+                int tt = j % tb, e2 = j / tb;
+                int tc = it.col0 + tt;
+                R v = (R)0;
+                if (q < cn && tc < T.B) {
+                    double c = T.cs[(long long)tc * T.ld_cs + c0 + q];
+                    if (e2 > 0) {
+                        double ri = bx.rho / T.r[c0 + q], rp = ri;
+                        for (int e = 1; e < e2; ++e) rp *= ri;
+                        c *= rp;
+                    }
+                    v = (R)c;
+                }
+                L.bop[q][j] = v;
             }
         }
         __syncthreads();
-        if (c0 + kMfQ < jr.y) prefetch(c0 + kMfQ);   // in flight during the products
-        int nsteps = (cn + 3) >> 2;
-        for (int st = 0; st < nsteps; ++st) {
-            int q = 4 * st + ka;
-            R ps = s_psi[q][pa];
-            C2 b = s_b[q];
-            R zr = ps * b.x, zi = ps * b.y;
-            R bval = s_c[q][tbk] * s_rp[q][e2b];
-#pragma unroll
-            for (int g = 0; g < GMAX; ++g) {
-                if (g < G) {
-                    int ea = (4 * g + e1a) * nsub;
-                    R rp = ea < E ? s_rp[q][ea] : (R)0;
-                    acc_r[g] = Mfma<R>::mma(zr * rp, bval, acc_r[g]);
-                    acc_i[g] = Mfma<R>::mma(zi * rp, bval, acc_i[g]);
-                }
-            }
-        }
+        const int qlo = max(qa, c0) - c0, qhi = min(qb, c0 + cn) - c0;
+        spread_chunk<R, EMAX, NG>(L, qlo, qhi, 4 * wv + pa, E, nsub, ka, e1a, jb, acc_r, acc_i);
         __syncthreads();
     }
-    // result element (row i = (p, e1), column j = (t, e2)) of group g
+    // ---- result element (row i = (p, e1), column j = (t, e2)) of group g
 #pragma unroll
-    for (int g = 0; g < GMAX; ++g) {
-        if (g >= G) continue;
+    for (int g = 0; g < NG; ++g) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             int i = Mfma<R>::row(lane, v), j = lane & 15;
-            int p = i >> 2, e1 = i & 3, t = it.col0 + j % tb, e2 = j / tb;
+            int p = 4 * wv + (i >> 2), e1 = i & 3, t = it.col0 + j % tb, e2 = j / tb;
             int e = (4 * g + e1) * nsub + e2;
             if (p >= it.count || e >= E || t >= T.B) continue;
             int pos = it.start + p;
@@ -708,6 +793,38 @@ __global__ __launch_bounds__(64) void k_asy_spread_mfma(const TfDesc *__restrict
                 w.x = vr;
                 w.y = vi;
                 *d = w;
+            }
+        }
+    }
+}
+
+template <typename R, int EMAX>
+__global__ __launch_bounds__(256) void k_asy_spread_mfma(const TfDesc *__restrict__ tfs,
+                                                         const AsyBoxD *__restrict__ boxes,
+                                                         const AsyItem *__restrict__ items,
+                                                         const SpreadRange *__restrict__ ranges,
+                                                         typename Cplx<R>::T *__restrict__ grid,
+                                                         const double *__restrict__ ft)
+{
+    __shared__ SpreadLds<R, EMAX> L;
+    const AsyItem it = items[blockIdx.x];
+    const int2 piece = ranges[blockIdx.x].piece;
+    const int2 trg = ranges[blockIdx.x].tile[threadIdx.x >> 6];
+    const AsyBoxD bx = boxes[it.box];
+    const TfDesc &T = tfs[bx.tf];
+    const int nsub = 16 / it.tb;
+    const int G = (bx.E + 4 * nsub - 1) / (4 * nsub);
+    // the group count is uniform per block: one branch-free body per count
+    constexpr int GMAX = (EMAX + 3) / 4;
+    if (G <= 1) spread_mfma_body<R, EMAX, 1>(T, bx, it, piece, trg, grid, ft, L);
+    else if (G == 2 || GMAX == 2) spread_mfma_body<R, EMAX, 2>(T, bx, it, piece, trg, grid, ft, L);
+    else if constexpr (GMAX >= 3) {
+        if (G == 3 || GMAX == 3) spread_mfma_body<R, EMAX, 3>(T, bx, it, piece, trg, grid, ft, L);
+        else if constexpr (GMAX >= 4) {
+            if (G == 4 || GMAX == 4) spread_mfma_body<R, EMAX, 4>(T, bx, it, piece, trg, grid, ft, L);
+            else if constexpr (GMAX >= 6) {
+                if (G == 5) spread_mfma_body<R, EMAX, 5>(T, bx, it, piece, trg, grid, ft, L);
+                else spread_mfma_body<R, EMAX, 6>(T, bx, it, piece, trg, grid, ft, L);
             }
         }
     }
