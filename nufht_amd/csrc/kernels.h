@@ -635,11 +635,14 @@ constexpr int kSP = 16;           // grid positions per block
 
 struct SpreadRange { int2 piece; int2 tile[4]; };
 
+// Row strides are padded so that both the staging writes (one source per
+// lane) and the product reads (4 sources x 4 positions or exponents per
+// instruction) spread over the LDS banks.
 template <typename R, int EMAX> struct SpreadLds {
     typedef typename Cplx<R>::T C2;
-    R psi[kSQ][kSP];              // kernel value of source q at the block's positions
-    C2 u[kSQ][EMAX];              // e^{iDx} r^{-1/2} (rho/r)^e
-    R bop[kSQ][16];               // B operand: c[q][t] (rho/r)^{e2}, column j = (t, e2)
+    R psi[kSQ][kSP + 5];          // kernel value of source q at the block's positions
+    C2 u[kSQ][EMAX + 1];          // e^{iDx} r^{-1/2} (rho/r)^e
+    R bop[kSQ][17];               // B operand: c[q][t] (rho/r)^{e2}, column j = (t, e2)
 };
 
 // Products of one staged chunk for NG exponent groups.  Sources outside a
