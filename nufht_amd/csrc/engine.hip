@@ -25,6 +25,7 @@
 
 #include "internal.h"
 #include "kernels.h"
+#include "fft.h"
 
 using namespace nufht_dev;
 
@@ -255,14 +256,88 @@ static double g_dense_cell = [] {
 }();
 
 // FFT lengths restricted to 2^a and 3*2^a (prebuilt rocFFT kernels)
+static int g_fft_pow2 = [] {
+    const char *e = getenv("NUFHT_AMD_FFT_POW2");
+    return e ? atoi(e) : 0;
+}();
 static int fft_length(long long need)
 {
     long long best = 1LL << 40;
     for (long long p = 8; p < (1LL << 34); p <<= 1) {
         if (p >= need) { best = std::min(best, p); break; }
-        if (3 * p >= need) best = std::min(best, 3 * p);
+        if (!g_fft_pow2 && 3 * p >= need) best = std::min(best, 3 * p);
     }
     return (int)best;
+}
+
+// Two-pass factorisation Nf = N1 N2 for the FFT kernels (Nf = 2^a or 3 2^a
+// above 4096; the factor 3 goes into N1), sequences per block and radices.
+static void fft_radices(int N, FftPass &p)
+{
+    p.nstage = 0;
+    int three = N % 3 == 0;
+    if (three) N /= 3;
+    int k = 0;
+    while ((1 << k) < N) ++k;
+    for (int q = 0; q < k / 3; ++q) p.radix[p.nstage++] = 8;
+    if (k % 3 == 2) p.radix[p.nstage++] = 4;
+    if (k % 3 == 1) p.radix[p.nstage++] = 2;
+    if (three) p.radix[p.nstage++] = 3;
+}
+static int fft_pick_m(int N, int other)
+{
+    int m = 16;
+    while (m > 1 && (m * N > kFftMaxElems || other % m)) m >>= 1;
+    return m;
+}
+static bool fft_factor(int nf, FftPass &p1, FftPass &p2)
+{
+    if (nf <= 4096) return false;
+    int b3 = nf % 3 == 0, q = b3 ? nf / 3 : nf, a = 0;
+    while ((1 << a) < q) ++a;
+    if ((1 << a) != q) return false;
+    // N2 = 2^j; the factorisation with the widest contiguous runs (min M)
+    int bestN2 = 0, bestScore = -1;
+    for (int j = 4; j <= 10; ++j) {
+        int N2 = 1 << j;
+        if (nf % N2) continue;
+        int N1 = nf / N2;
+        if (N1 < 16 || N1 > 1536) continue;
+        int m1 = fft_pick_m(N1, N2), m2 = fft_pick_m(N2, N1);
+        int score = 64 * std::min(m1, m2) + std::max(m1, m2);
+        if (score > bestScore) { bestScore = score; bestN2 = N2; }
+    }
+    if (!bestN2) return false;
+    int N2 = bestN2, N1 = nf / N2;
+    for (FftPass *p : {&p1, &p2}) {
+        p->N1 = N1;
+        p->N2 = N2;
+        p->nf = nf;
+    }
+    p1.N = N1;
+    p1.M = fft_pick_m(N1, N2);
+    p1.tiles = N2 / p1.M;
+    p2.N = N2;
+    p2.M = fft_pick_m(N2, N1);
+    p2.tiles = N1 / p2.M;
+    for (FftPass *p : {&p1, &p2}) {
+        p->lgM = 0;
+        while ((1 << p->lgM) < p->M) ++p->lgM;
+        p->lgN = 0;
+        while ((1 << p->lgN) < p->N) ++p->lgN;
+    }
+    fft_radices(N1, p1);
+    fft_radices(N2, p2);
+    return p1.M >= 4 && p2.M >= 4;
+}
+static void fft_table(std::vector<double> &tw, int n, int count, int *off)
+{
+    *off = (int)(tw.size() / 2);
+    for (int m = 0; m < count; ++m) {
+        double a = 2.0 * M_PI * (double)m / (double)n;
+        tw.push_back(std::cos(a));
+        tw.push_back(std::sin(a));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -275,6 +350,9 @@ struct DevPlan {
 struct FftClass {
     int nf = 0;
     long long batch = 0, off = 0;
+    bool own = false;          // two-pass FFT kernels (else rocFFT, out of place)
+    FftPass p1, p2;
+    size_t lds1 = 0, lds2 = 0;
     rocfft_plan plan = nullptr;
     rocfft_execution_info info = nullptr;
     void *work = nullptr;
@@ -308,7 +386,8 @@ struct Program {
     int2 *d_arange = nullptr;
     SpreadRange *d_adrange = nullptr;
     int interp_tmax = 16;
-    void *d_grid = nullptr;
+    void *d_grid = nullptr, *d_grid2 = nullptr;   // spread grid; transformed grid
+    void *d_fftw = nullptr;                        // FFT twiddle tables
     long long grid_elems = 0;
     std::vector<FftClass> fft;
     double *d_ft = nullptr, *d_cheb = nullptr, *d_jtab = nullptr;
@@ -359,6 +438,7 @@ static void free_program(Program *P)
     hipFree(P->d_tfs); hipFree(P->d_zero); hipFree(P->d_dir); hipFree(P->d_lbox);
     hipFree(P->d_lproj); hipFree(P->d_leval); hipFree(P->d_coef); hipFree(P->d_abox);
     hipFree(P->d_aspread); hipFree(P->d_ainterp); hipFree(P->d_grid); hipFree(P->d_ft);
+    hipFree(P->d_grid2); hipFree(P->d_fftw);
     hipFree(P->d_azero); hipFree(P->d_arange); hipFree(P->d_adense); hipFree(P->d_adrange);
     hipFree(P->d_aimf);
     hipFree(P->d_cheb);
@@ -743,14 +823,65 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMalloc((void **)&P->d_tfs, std::max<size_t>(1, P->tfs.size()) * sizeof(TfDesc)));
     size_t esz = mixed ? sizeof(float2) : sizeof(double2);
-    if (P->grid_elems) HIP_TRY(hipMalloc(&P->d_grid, P->grid_elems * esz));
+    if (P->grid_elems) {
+        HIP_TRY(hipMalloc(&P->d_grid, P->grid_elems * esz));
+        HIP_TRY(hipMalloc(&P->d_grid2, P->grid_elems * esz));
+    }
+    // two-pass FFT classes: twiddle tables (in the grid's precision)
+    std::vector<double> tw;
     for (auto &c : P->fft) {
+        c.own = fft_factor(c.nf, c.p1, c.p2);
+        if (!c.own) continue;
+        fft_table(tw, c.p1.N, c.p1.N, &c.p1.w_off);
+        fft_table(tw, c.p2.N, c.p2.N, &c.p2.w_off);
+        // w^a = hi[a >> 8] lo[a & 255], w = e^{2 pi i / Nf}
+        c.p1.hi_off = (int)(tw.size() / 2);
+        for (int j = 0; j < (c.nf + 255) / 256; ++j) {
+            double a = 2.0 * M_PI * (256.0 * j) / (double)c.nf;
+            tw.push_back(std::cos(a));
+            tw.push_back(std::sin(a));
+        }
+        fft_table(tw, c.nf, 256, &c.p1.lo_off);
+        c.p2.hi_off = c.p1.hi_off;
+        c.p2.lo_off = c.p1.lo_off;
+        c.lds1 = fft_lds_elems(c.p1.N, c.p1.M) * esz;
+        c.lds2 = fft_lds_elems(c.p2.N, c.p2.M) * esz;
+    }
+    if (!tw.empty()) {
+        if (mixed) {
+            std::vector<float> twf(tw.begin(), tw.end());
+            RC_TRY(upload((float **)&P->d_fftw, twf));
+        } else {
+            RC_TRY(upload((double **)&P->d_fftw, tw));
+        }
+        static bool attr_done = false;
+        if (!attr_done) {
+            hipFuncSetAttribute((const void *)k_fft_cols<double>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            hipFuncSetAttribute((const void *)k_fft_rows<double>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            hipFuncSetAttribute((const void *)k_fft_cols<float>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            hipFuncSetAttribute((const void *)k_fft_rows<float>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            attr_done = true;
+        }
+    }
+    for (auto &c : P->fft) {
+        if (c.own) continue;
         size_t len = (size_t)c.nf;
-        rocfft_status s = rocfft_plan_create(&c.plan, rocfft_placement_inplace,
+        rocfft_plan_description desc = nullptr;
+        rocfft_plan_description_create(&desc);
+        rocfft_plan_description_set_data_layout(desc, rocfft_array_type_complex_interleaved,
+                                                rocfft_array_type_complex_interleaved, nullptr,
+                                                nullptr, 0, nullptr, (size_t)c.nf, 0, nullptr,
+                                                (size_t)c.nf);
+        rocfft_status s = rocfft_plan_create(&c.plan, rocfft_placement_notinplace,
                                              rocfft_transform_type_complex_inverse,
                                              mixed ? rocfft_precision_single
                                                    : rocfft_precision_double,
-                                             1, &len, (size_t)c.batch, nullptr);
+                                             1, &len, (size_t)c.batch, desc);
+        rocfft_plan_description_destroy(desc);
         if (s != rocfft_status_success) {
             nufht_set_error("rocfft_plan_create(len=%d, batch=%lld) failed: %d", c.nf, c.batch, (int)s);
             return -520;
@@ -791,9 +922,18 @@ static void launch_asy(Program *P, hipStream_t st)
     {
         StageTimer tm(ST_FFT, st);
         for (auto &c : P->fft) {
+            G *in = (G *)P->d_grid + c.off, *out = (G *)P->d_grid2 + c.off;
+            if (c.own) {
+                const G *tw = (const G *)P->d_fftw;
+                hipLaunchKernelGGL((k_fft_cols<R>), dim3((unsigned)(c.batch * c.p1.tiles)),
+                                   dim3(kFftThreads), c.lds1, st, in, c.p1, tw);
+                hipLaunchKernelGGL((k_fft_rows<R>), dim3((unsigned)(c.batch * c.p2.tiles)),
+                                   dim3(kFftThreads), c.lds2, st, (const G *)in, out, c.p2, tw);
+                continue;
+            }
             rocfft_execution_info_set_stream(c.info, st);
-            void *buf = (void *)((G *)P->d_grid + c.off);
-            rocfft_execute(c.plan, &buf, nullptr, c.info);
+            void *ib = (void *)in, *ob = (void *)out;
+            rocfft_execute(c.plan, &ib, &ob, c.info);
         }
     }
     if (!P->ainterp.empty()) {
@@ -801,13 +941,13 @@ static void launch_asy(Program *P, hipStream_t st)
         int tmx = P->interp_tmax;
         size_t lds = interp_lds_off(tmx, 2, sizeof(R)) * sizeof(double);
         hipLaunchKernelGGL((k_asy_interp<R, EM>), dim3((unsigned)P->ainterp.size()), dim3(256),
-                           lds, st, P->d_tfs, P->d_abox, P->d_ainterp, (const G *)P->d_grid,
+                           lds, st, P->d_tfs, P->d_abox, P->d_ainterp, (const G *)P->d_grid2,
                            P->d_ft, tmx);
     }
     if (!P->aimf.empty()) {
         StageTimer tm(ST_INTERP_MF, st);
         hipLaunchKernelGGL((k_asy_interp_mfma<R, EM>), dim3((unsigned)P->aimf.size()), dim3(64),
-                           0, st, P->d_tfs, P->d_abox, P->d_aimf, (const G *)P->d_grid, P->d_ft);
+                           0, st, P->d_tfs, P->d_abox, P->d_aimf, (const G *)P->d_grid2, P->d_ft);
     }
 }
 

@@ -1,0 +1,230 @@
+// Large batched complex FFTs of the asymptotic stage's grids, written for
+// gfx950: the backward transform (e^{+2 pi i k n / Nf}, unnormalised) of
+// every channel of an FFT size class, Nf = N1 * N2, in two passes over HBM
+// (the four-step decomposition):
+//
+//   pass 1 (in place):     for each column n2, X1[k1][n2] = w^{k1 n2} *
+//                          sum_n1 x[n1 N2 + n2] e^{2 pi i k1 n1 / N1}
+//                          with w = e^{2 pi i / Nf}, stored at k1 N2 + n2;
+//   pass 2 (out of place): for each row k1, X[k1 + N1 k2] =
+//                          sum_n2 X1[k1][n2] e^{2 pi i k2 n2 / N2}.
+//
+// A block holds M whole sequences (M adjacent columns, or M adjacent rows)
+// in LDS, so that global reads and writes move runs of M contiguous
+// elements; within LDS each sequence is transformed by a Stockham radix-
+// 2/3/4/8 FFT whose butterflies are held in registers across one barrier
+// per stage (a single LDS buffer).
+#pragma once
+
+namespace nufht_dev {
+
+constexpr int kFftThreads = 512;
+constexpr int kFftMaxElems = 8192;    // M * N per block
+constexpr int kFftMaxStages = 8;
+
+struct FftPass {
+    int N, M;                 // transform length; sequences per block (a power of 2)
+    int lgM, lgN;             // log2 M; log2 N when N is a power of 2
+    int N1, N2;               // the class's factorisation, Nf = N1 * N2
+    int tiles;                // blocks per channel (columns or rows / M)
+    int nstage;
+    int radix[kFftMaxStages]; // powers of 2 first, then at most one 3
+    int w_off;                // W_N[m] = e^{2 pi i m / N}, m < N
+    int hi_off, lo_off;       // pass 1: w^a = hi[a >> 8] lo[a & 255], a < Nf
+    long long nf;
+};
+
+template <typename R> struct CMul {
+    typedef typename Cplx<R>::T C2;
+    static __device__ __forceinline__ C2 mul(C2 a, C2 b)
+    {
+        C2 z;
+        z.x = a.x * b.x - a.y * b.y;
+        z.y = a.x * b.y + a.y * b.x;
+        return z;
+    }
+    static __device__ __forceinline__ C2 add(C2 a, C2 b) { a.x += b.x; a.y += b.y; return a; }
+    static __device__ __forceinline__ C2 sub(C2 a, C2 b) { a.x -= b.x; a.y -= b.y; return a; }
+    static __device__ __forceinline__ C2 mul_i(C2 a) { C2 z; z.x = -a.y; z.y = a.x; return z; }
+};
+
+// small DFTs with the backward sign
+template <typename R, int RAD> struct Dft;
+template <typename R> struct Dft<R, 2> {
+    typedef typename Cplx<R>::T C2;
+    static __device__ __forceinline__ void run(C2 *x)
+    {
+        C2 a = x[0], b = x[1];
+        x[0] = CMul<R>::add(a, b);
+        x[1] = CMul<R>::sub(a, b);
+    }
+};
+template <typename R> struct Dft<R, 3> {
+    typedef typename Cplx<R>::T C2;
+    static __device__ __forceinline__ void run(C2 *x)
+    {
+        const R h = (R)0.86602540378443864676;   // sqrt(3) / 2
+        C2 t = CMul<R>::add(x[1], x[2]), d = CMul<R>::sub(x[1], x[2]);
+        C2 m;
+        m.x = x[0].x - (R)0.5 * t.x;
+        m.y = x[0].y - (R)0.5 * t.y;
+        x[0] = CMul<R>::add(x[0], t);
+        C2 id;                                    // i (sqrt(3)/2) (x1 - x2)
+        id.x = -h * d.y;
+        id.y = h * d.x;
+        x[1] = CMul<R>::add(m, id);
+        x[2] = CMul<R>::sub(m, id);
+    }
+};
+template <typename R> struct Dft<R, 4> {
+    typedef typename Cplx<R>::T C2;
+    static __device__ __forceinline__ void run(C2 *x)
+    {
+        C2 a = CMul<R>::add(x[0], x[2]), b = CMul<R>::sub(x[0], x[2]);
+        C2 c = CMul<R>::add(x[1], x[3]), d = CMul<R>::mul_i(CMul<R>::sub(x[1], x[3]));
+        x[0] = CMul<R>::add(a, c);
+        x[2] = CMul<R>::sub(a, c);
+        x[1] = CMul<R>::add(b, d);
+        x[3] = CMul<R>::sub(b, d);
+    }
+};
+template <typename R> struct Dft<R, 8> {
+    typedef typename Cplx<R>::T C2;
+    static __device__ __forceinline__ void run(C2 *x)
+    {
+        const R s = (R)0.70710678118654752440;   // sqrt(2) / 2
+        C2 e[4] = {x[0], x[2], x[4], x[6]}, o[4] = {x[1], x[3], x[5], x[7]};
+        Dft<R, 4>::run(e);
+        Dft<R, 4>::run(o);
+        // o[k] *= e^{i pi k / 4}
+        C2 t;
+        t.x = s * (o[1].x - o[1].y);
+        t.y = s * (o[1].x + o[1].y);
+        o[1] = t;
+        o[2] = CMul<R>::mul_i(o[2]);
+        t.x = -s * (o[3].x + o[3].y);
+        t.y = s * (o[3].x - o[3].y);
+        o[3] = t;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            x[k] = CMul<R>::add(e[k], o[k]);
+            x[k + 4] = CMul<R>::sub(e[k], o[k]);
+        }
+    }
+};
+
+// LDS position of element n of sequence c: rows of M, one pad element every
+// 8 rows, so that the stride-R writes of a first stage spread over banks.
+__device__ __forceinline__ int fft_lds_idx(int n, int c, int M) { return n * M + c + (n >> 3); }
+__host__ __device__ inline size_t fft_lds_elems(int N, int M) { return (size_t)N * M + (N >> 3) + M; }
+
+// One Stockham stage of radix RAD over M sequences of length N held in LDS
+// at fft_lds_idx(n, c); Ns = product of the earlier radices.
+// Ns is a power of 2 (lgNs its log): the radix-3 stage, if any, comes last.
+template <typename R, int RAD>
+__device__ __forceinline__ void fft_stage(typename Cplx<R>::T *s, int N, int M, int lgM, int lgNs,
+                                          const typename Cplx<R>::T *__restrict__ W)
+{
+    typedef typename Cplx<R>::T C2;
+    constexpr int MAXB = (kFftMaxElems / RAD + kFftThreads - 1) / kFftThreads;
+    const int Ns = 1 << lgNs;
+    const int nb = M * N / RAD, span = N / RAD;
+    const int wstep = N / (Ns * RAD);
+    C2 x[MAXB][RAD];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        int b = threadIdx.x + q * kFftThreads;
+        if (b < nb) {
+            int c = b & (M - 1), j = b >> lgM, k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) x[q][r] = s[fft_lds_idx(j + r * span, c, M)];
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < RAD; ++r) x[q][r] = CMul<R>::mul(x[q][r], W[r * k * wstep]);
+            }
+            Dft<R, RAD>::run(x[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        int b = threadIdx.x + q * kFftThreads;
+        if (b < nb) {
+            int c = b & (M - 1), j = b >> lgM, k = j & (Ns - 1);
+            int base = (j >> lgNs) * Ns * RAD + k;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) s[fft_lds_idx(base + r * Ns, c, M)] = x[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename R>
+__device__ __forceinline__ void fft_lds(typename Cplx<R>::T *s, const FftPass &P,
+                                        const typename Cplx<R>::T *__restrict__ W)
+{
+    int lgNs = 0;
+    for (int st = 0; st < P.nstage; ++st) {
+        int rad = P.radix[st];
+        if (rad == 8) fft_stage<R, 8>(s, P.N, P.M, P.lgM, lgNs, W);
+        else if (rad == 4) fft_stage<R, 4>(s, P.N, P.M, P.lgM, lgNs, W);
+        else if (rad == 3) fft_stage<R, 3>(s, P.N, P.M, P.lgM, lgNs, W);
+        else fft_stage<R, 2>(s, P.N, P.M, P.lgM, lgNs, W);
+        lgNs += rad == 8 ? 3 : rad == 4 ? 2 : 1;
+    }
+}
+
+// Pass 1: column FFTs of length N1 (stride N2) in place, times w^{k1 n2}.
+template <typename R>
+__global__ __launch_bounds__(kFftThreads) void k_fft_cols(typename Cplx<R>::T *__restrict__ g,
+                                                          FftPass P,
+                                                          const typename Cplx<R>::T *__restrict__ tw)
+{
+    typedef typename Cplx<R>::T C2;
+    extern __shared__ unsigned char s_raw[];
+    C2 *s = (C2 *)s_raw;
+    const int ch = blockIdx.x / P.tiles, t = blockIdx.x % P.tiles;
+    const int N = P.N, M = P.M;
+    C2 *base = g + (long long)ch * P.nf + (long long)t * M;
+    for (int idx = threadIdx.x; idx < M * N; idx += kFftThreads) {
+        int c = idx & (M - 1), n = idx >> P.lgM;
+        s[fft_lds_idx(n, c, M)] = base[(long long)n * P.N2 + c];
+    }
+    __syncthreads();
+    fft_lds<R>(s, P, tw + P.w_off);
+    const C2 *hi = tw + P.hi_off, *lo = tw + P.lo_off;
+    for (int idx = threadIdx.x; idx < M * N; idx += kFftThreads) {
+        int c = idx & (M - 1), k1 = idx >> P.lgM;
+        int a = k1 * (t * M + c);          // < (N1 - 1) (N2 - 1) < Nf
+        C2 w = CMul<R>::mul(hi[a >> 8], lo[a & 255]);
+        base[(long long)k1 * P.N2 + c] = CMul<R>::mul(s[fft_lds_idx(k1, c, M)], w);
+    }
+}
+
+// Pass 2: row FFTs of length N2, written to natural order in out.
+template <typename R>
+__global__ __launch_bounds__(kFftThreads) void k_fft_rows(const typename Cplx<R>::T *__restrict__ g,
+                                                          typename Cplx<R>::T *__restrict__ out,
+                                                          FftPass P,
+                                                          const typename Cplx<R>::T *__restrict__ tw)
+{
+    typedef typename Cplx<R>::T C2;
+    extern __shared__ unsigned char s_raw[];
+    C2 *s = (C2 *)s_raw;
+    const int ch = blockIdx.x / P.tiles, t = blockIdx.x % P.tiles;
+    const int N = P.N, M = P.M;
+    const C2 *src = g + (long long)ch * P.nf + (long long)t * M * P.N2;
+    for (int idx = threadIdx.x; idx < M * N; idx += kFftThreads) {
+        int n = idx & (N - 1), c = idx >> P.lgN;     // N = N2, a power of 2
+        s[fft_lds_idx(n, c, M)] = src[(long long)c * P.N2 + n];
+    }
+    __syncthreads();
+    fft_lds<R>(s, P, tw + P.w_off);
+    C2 *dst = out + (long long)ch * P.nf + (long long)t * M;
+    for (int idx = threadIdx.x; idx < M * N; idx += kFftThreads) {
+        int c = idx & (M - 1), k2 = idx >> P.lgM;
+        dst[(long long)k2 * P.N1 + c] = s[fft_lds_idx(k2, c, M)];
+    }
+}
+
+}  // namespace nufht_dev
