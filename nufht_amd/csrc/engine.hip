@@ -248,7 +248,7 @@ static const int kJTabMax = 8192;
 static const int kSpreadSplit = 2048;
 static const int kDenseSplit = 4096;
 // sources per grid cell from which a box uses the split-K spread kernel
-static const double kDenseCell = 32.0;
+static const double kDenseCell = 4.0;
 // (NUFHT_AMD_DENSE_CELL overrides the threshold, for tuning)
 static double g_dense_cell = [] {
     const char *e = getenv("NUFHT_AMD_DENSE_CELL");
@@ -388,6 +388,8 @@ struct Program {
     int interp_tmax = 16;
     void *d_grid = nullptr, *d_grid2 = nullptr;   // spread grid; transformed grid
     void *d_fftw = nullptr;                        // FFT twiddle tables
+    std::vector<int4> fft_chinfo;                  // per channel of own FFT classes: L, mlo, span
+    int4 *d_fft_chinfo = nullptr;
     long long grid_elems = 0;
     std::vector<FftClass> fft;
     double *d_ft = nullptr, *d_cheb = nullptr, *d_jtab = nullptr;
@@ -438,7 +440,7 @@ static void free_program(Program *P)
     hipFree(P->d_tfs); hipFree(P->d_zero); hipFree(P->d_dir); hipFree(P->d_lbox);
     hipFree(P->d_lproj); hipFree(P->d_leval); hipFree(P->d_coef); hipFree(P->d_abox);
     hipFree(P->d_aspread); hipFree(P->d_ainterp); hipFree(P->d_grid); hipFree(P->d_ft);
-    hipFree(P->d_grid2); hipFree(P->d_fftw);
+    hipFree(P->d_grid2); hipFree(P->d_fftw); hipFree(P->d_fft_chinfo);
     hipFree(P->d_azero); hipFree(P->d_arange); hipFree(P->d_adense); hipFree(P->d_adrange);
     hipFree(P->d_aimf);
     hipFree(P->d_cheb);
@@ -660,10 +662,14 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     std::map<int, std::vector<int>> byNf;
     for (size_t q = 0; q < apend.size(); ++q) byNf[apend[q].b.Nf].push_back((int)q);
     long long off = 0;
+    std::map<int, bool> own_nf;
     for (auto &kv : byNf) {
         FftClass c;
         c.nf = kv.first;
         c.off = off;
+        c.own = fft_factor(c.nf, c.p1, c.p2);
+        own_nf[c.nf] = c.own;
+        c.p1.chinfo_off = c.p2.chinfo_off = (int)P->fft_chinfo.size();
         for (int q : kv.second) {
             AsyBoxD &A = apend[q].b;
             const TfDesc &T = P->tfs[A.tf];
@@ -671,6 +677,17 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
             long long ch = (long long)A.E * T.B;
             off += ch * A.Nf;
             c.batch += ch;
+            if (c.own) {
+                // the transformed grid is read only at frequencies [mlo, mhi]
+                // (mod Nf) by the interpolation of the box's targets
+                const NufhtImplHost *h = (const NufhtImplHost *)in[A.tf].plan->impl;
+                double smin = h->w[A.i0] - A.D, smax = h->w[A.i0 + A.ni - 1] - A.D;
+                int mlo = (int)std::ceil((smin - A.a2) / A.delta) - 4;
+                int mhi = (int)std::ceil((smax - A.a2) / A.delta) + T.kw + 4;
+                if (mhi - mlo + 1 >= A.Nf) { mlo = 0; mhi = A.Nf - 1; }
+                int4 info = make_int4(A.L, mlo, mhi - mlo, 0);
+                for (long long k = 0; k < ch; ++k) P->fft_chinfo.push_back(info);
+            }
         }
         P->fft.push_back(c);
     }
@@ -706,9 +723,19 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         for (int c0 = 0; c0 < T.B; c0 += tb) {
             long long chans = (long long)A.E * std::min(tb, T.B - c0);
             (dense ? P->stats.spread_sc_mfma : P->stats.spread_sc_valu) += (long long)A.nj * chans;
+            // position ranges to spread: the whole grid, or for the two-pass
+            // FFT (which reads only l in [-L, L]) just [0, L] and [Nf - L, Nf)
+            std::vector<std::pair<int, int>> pr;
+            if (own_nf[A.Nf]) {
+                pr.push_back({0, A.L + 1});
+                pr.push_back({A.Nf - A.L, A.Nf});
+            } else {
+                pr.push_back({0, A.Nf});
+            }
             if (dense) {
-                for (int s = 0; s < A.Nf; s += kSP) {
-                    int np = std::min(kSP, A.Nf - s);
+                for (auto &rg : pr)
+                for (int s = rg.first; s < rg.second; s += kSP) {
+                    int np = std::min(kSP, rg.second - s);
                     SpreadRange sr;
                     sr.piece = src_range(s, np);
                     for (int t = 0; t < 4; ++t)
@@ -728,8 +755,9 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                     }
                 }
             } else {
-                for (int s = 0; s < A.Nf; s += sp) {
-                    int np = std::min(sp, A.Nf - s);
+                for (auto &rg : pr)
+                for (int s = rg.first; s < rg.second; s += sp) {
+                    int np = std::min(sp, rg.second - s);
                     int2 jr = src_range(s, np);
                     if (jr.y - jr.x <= kSpreadSplit) {
                         P->aspread.push_back({box, s, np, c0, tb, 0});
@@ -830,7 +858,6 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     // two-pass FFT classes: twiddle tables (in the grid's precision)
     std::vector<double> tw;
     for (auto &c : P->fft) {
-        c.own = fft_factor(c.nf, c.p1, c.p2);
         if (!c.own) continue;
         fft_table(tw, c.p1.N, c.p1.N, &c.p1.w_off);
         fft_table(tw, c.p2.N, c.p2.N, &c.p2.w_off);
@@ -847,6 +874,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         c.lds1 = fft_lds_elems(c.p1.N, c.p1.M) * esz;
         c.lds2 = fft_lds_elems(c.p2.N, c.p2.M) * esz;
     }
+    if (!P->fft_chinfo.empty()) RC_TRY(upload(&P->d_fft_chinfo, P->fft_chinfo));
     if (!tw.empty()) {
         if (mixed) {
             std::vector<float> twf(tw.begin(), tw.end());
@@ -926,9 +954,11 @@ static void launch_asy(Program *P, hipStream_t st)
             if (c.own) {
                 const G *tw = (const G *)P->d_fftw;
                 hipLaunchKernelGGL((k_fft_cols<R>), dim3((unsigned)(c.batch * c.p1.tiles)),
-                                   dim3(kFftThreads), c.lds1, st, in, c.p1, tw);
+                                   dim3(kFftThreads), c.lds1, st, in, c.p1, tw,
+                                   (const int4 *)P->d_fft_chinfo);
                 hipLaunchKernelGGL((k_fft_rows<R>), dim3((unsigned)(c.batch * c.p2.tiles)),
-                                   dim3(kFftThreads), c.lds2, st, (const G *)in, out, c.p2, tw);
+                                   dim3(kFftThreads), c.lds2, st, (const G *)in, out, c.p2, tw,
+                                   (const int4 *)P->d_fft_chinfo);
                 continue;
             }
             rocfft_execution_info_set_stream(c.info, st);

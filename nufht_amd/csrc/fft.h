@@ -31,6 +31,9 @@ struct FftPass {
     int radix[kFftMaxStages]; // powers of 2 first, then at most one 3
     int w_off;                // W_N[m] = e^{2 pi i m / N}, m < N
     int hi_off, lo_off;       // pass 1: w^a = hi[a >> 8] lo[a & 255], a < Nf
+    int chinfo_off;           // per channel {L, mlo, span}: pass 1 reads only grid positions
+                              // l in [-L, L] (the rest are zeros not stored), pass 2
+                              // writes only frequencies mlo .. mlo + span (mod Nf)
     long long nf;
 };
 
@@ -178,17 +181,25 @@ __device__ __forceinline__ void fft_lds(typename Cplx<R>::T *s, const FftPass &P
 template <typename R>
 __global__ __launch_bounds__(kFftThreads) void k_fft_cols(typename Cplx<R>::T *__restrict__ g,
                                                           FftPass P,
-                                                          const typename Cplx<R>::T *__restrict__ tw)
+                                                          const typename Cplx<R>::T *__restrict__ tw,
+                                                          const int4 *__restrict__ chinfo)
 {
     typedef typename Cplx<R>::T C2;
     extern __shared__ unsigned char s_raw[];
     C2 *s = (C2 *)s_raw;
     const int ch = blockIdx.x / P.tiles, t = blockIdx.x % P.tiles;
     const int N = P.N, M = P.M;
+    const int L = chinfo[P.chinfo_off + ch].x;
+    const int nf = (int)P.nf;
     C2 *base = g + (long long)ch * P.nf + (long long)t * M;
     for (int idx = threadIdx.x; idx < M * N; idx += kFftThreads) {
         int c = idx & (M - 1), n = idx >> P.lgM;
-        s[fft_lds_idx(n, c, M)] = base[(long long)n * P.N2 + c];
+        int p = n * P.N2 + t * M + c;
+        C2 v;
+        v.x = (R)0;
+        v.y = (R)0;
+        if (p <= L || p >= nf - L) v = base[(long long)n * P.N2 + c];
+        s[fft_lds_idx(n, c, M)] = v;
     }
     __syncthreads();
     fft_lds<R>(s, P, tw + P.w_off);
@@ -206,7 +217,8 @@ template <typename R>
 __global__ __launch_bounds__(kFftThreads) void k_fft_rows(const typename Cplx<R>::T *__restrict__ g,
                                                           typename Cplx<R>::T *__restrict__ out,
                                                           FftPass P,
-                                                          const typename Cplx<R>::T *__restrict__ tw)
+                                                          const typename Cplx<R>::T *__restrict__ tw,
+                                                          const int4 *__restrict__ chinfo)
 {
     typedef typename Cplx<R>::T C2;
     extern __shared__ unsigned char s_raw[];
@@ -221,9 +233,14 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_rows(const typename Cplx<R>
     __syncthreads();
     fft_lds<R>(s, P, tw + P.w_off);
     C2 *dst = out + (long long)ch * P.nf + (long long)t * M;
+    const int4 ci = chinfo[P.chinfo_off + ch];
+    const int nf = (int)P.nf;
     for (int idx = threadIdx.x; idx < M * N; idx += kFftThreads) {
         int c = idx & (M - 1), k2 = idx >> P.lgM;
-        dst[(long long)k2 * P.N1 + c] = s[fft_lds_idx(k2, c, M)];
+        int q = t * M + c + k2 * P.N1;
+        int d = q - ci.y;                  // mlo in (-Nf, Nf)
+        d = d < 0 ? d + nf : (d >= nf ? d - nf : d);
+        if (d <= ci.z) dst[(long long)k2 * P.N1 + c] = s[fft_lds_idx(k2, c, M)];
     }
 }
 

@@ -136,6 +136,21 @@ __device__ __forceinline__ R es_tap(const double *__restrict__ c, const R (&t)[k
     return s;
 }
 
+// the same tap by Clenshaw's recurrence (no basis array: few registers)
+template <typename R>
+__device__ __forceinline__ R es_tap_cl(const double *__restrict__ c, R y)
+{
+    R b1 = (R)0, b2 = (R)0;
+    const R y2 = y + y;
+#pragma unroll
+    for (int n = kEsN - 1; n >= 1; --n) {
+        R b0 = fma(y2, b1, (R)c[n] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    return fma(y, b1, (R)c[0] - b2);
+}
+
 __device__ __forceinline__ double es_kernel(double u, double beta)
 {
     double t = 1.0 - u * u;
@@ -494,7 +509,6 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
     int t0 = it.col0 + 2 * tl;
     bool active = valid && (t0 < T.B);
     int kw = T.kw;
-    double beta = T.beta;
     double inv_a1 = 1.0 / bx.a1;
     R a0r[EMAX], a0i[EMAX], a1r[EMAX], a1i[EMAX];
 #pragma unroll
@@ -506,12 +520,13 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
             double rj = T.r[c0 + q];
             double x = rj - bx.C;
             double sn, cs;
-            sincos(bx.D * x, &sn, &cs);
+            sincos_cw(bx.D * x, &sn, &cs);
             double rs = 1.0 / sqrt(rj);
+            int lf = (int)ceil((x - bx.a1) / bx.h);
             s_x[q] = x;
             s_br[q] = cs * rs;
             s_bi[q] = sn * rs;
-            s_pf[q] = (int)ceil((x - bx.a1) / bx.h);
+            s_pf[q] = lf;
             s_ri[q] = (R)(bx.rho / rj);
         }
         for (int q = threadIdx.x; q < cn * 8; q += 256) {
@@ -528,7 +543,7 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
             double psi = 0.0;
             if (k <= kw) {
                 double u = ((double)(s_pf[jq] + k) * bx.h - s_x[jq]) * inv_a1;
-                psi = (double)es_eval((R)u, (R)beta);
+                psi = (double)es_eval((R)u, (R)T.beta);
             }
             C2 z;
             z.x = (R)(psi * s_br[jq]);
@@ -840,14 +855,17 @@ __global__ __launch_bounds__(256) void k_asy_spread_mfma(const TfDesc *__restric
 //   D[i][(t, e2)] = sum_m phi(s_i - m Delta) grid[e][t][m],  e = g nsub + e2
 // with the phi matrix computed once per tile; the FP64 epilogue applies
 // e^{i(phi0 + w C)} h Delta / psihat(s) and the Hankel weights.
-template <typename R, int EMAX>
+// (GR: the grid's precision; the products run in FP64 either way, which on
+// this hardware is faster than the FP32 form of this kernel)
+template <typename GR, int EMAX>
 __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict__ tfs,
                                                         const AsyBoxD *__restrict__ boxes,
                                                         const AsyItem *__restrict__ items,
-                                                        const typename Cplx<R>::T *__restrict__ grid,
+                                                        const typename Cplx<GR>::T *__restrict__ grid,
                                                         const double *__restrict__ ft)
 {
-    typedef typename Cplx<R>::T C2;
+    typedef double R;
+    typedef typename Cplx<GR>::T C2;
     typedef typename Mfma<R>::V V;
     __shared__ double s_s[16], s_fr[16], s_fi[16], s_b0[16];
     __shared__ double s_pw[16][EMAX];
@@ -910,11 +928,11 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
                 int m = mw0 + 4 * st + ka;
                 m = m < 0 ? m + bx.Nf : (m >= bx.Nf ? m - bx.Nf : m);
                 C2 b;
-                b.x = (R)0;
-                b.y = (R)0;
+                b.x = (GR)0;
+                b.y = (GR)0;
                 if (eok) b = row[m];
-                dr = Mfma<R>::mma(phiA[st], b.x, dr);
-                di = Mfma<R>::mma(phiA[st], b.y, di);
+                dr = Mfma<R>::mma(phiA[st], (R)b.x, dr);
+                di = Mfma<R>::mma(phiA[st], (R)b.y, di);
             }
         }
 #pragma unroll
