@@ -179,7 +179,7 @@ __device__ __forceinline__ void fft_lds(typename Cplx<R>::T *s, const FftPass &P
 
 // Pass 1: column FFTs of length N1 (stride N2) in place, times w^{k1 n2}.
 template <typename R>
-__global__ __launch_bounds__(kFftThreads) void k_fft_cols(typename Cplx<R>::T *__restrict__ g,
+__global__ __launch_bounds__(kFftThreads, 1) void k_fft_cols(typename Cplx<R>::T *__restrict__ g,
                                                           FftPass P,
                                                           const typename Cplx<R>::T *__restrict__ tw,
                                                           const int4 *__restrict__ chinfo)
@@ -192,14 +192,22 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_cols(typename Cplx<R>::T *_
     const int L = chinfo[P.chinfo_off + ch].x;
     const int nf = (int)P.nf;
     C2 *base = g + (long long)ch * P.nf + (long long)t * M;
-    for (int idx = threadIdx.x; idx < M * N; idx += kFftThreads) {
+    // all of a thread's loads in flight at once, then into LDS
+    constexpr int EPT = kFftMaxElems / kFftThreads;
+    C2 v[EPT];
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+        int idx = threadIdx.x + r * kFftThreads;
         int c = idx & (M - 1), n = idx >> P.lgM;
         int p = n * P.N2 + t * M + c;
-        C2 v;
-        v.x = (R)0;
-        v.y = (R)0;
-        if (p <= L || p >= nf - L) v = base[(long long)n * P.N2 + c];
-        s[fft_lds_idx(n, c, M)] = v;
+        v[r].x = (R)0;
+        v[r].y = (R)0;
+        if (idx < M * N && (p <= L || p >= nf - L)) v[r] = base[(long long)n * P.N2 + c];
+    }
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+        int idx = threadIdx.x + r * kFftThreads;
+        if (idx < M * N) s[fft_lds_idx(idx >> P.lgM, idx & (M - 1), M)] = v[r];
     }
     __syncthreads();
     fft_lds<R>(s, P, tw + P.w_off);
@@ -214,7 +222,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_cols(typename Cplx<R>::T *_
 
 // Pass 2: row FFTs of length N2, written to natural order in out.
 template <typename R>
-__global__ __launch_bounds__(kFftThreads) void k_fft_rows(const typename Cplx<R>::T *__restrict__ g,
+__global__ __launch_bounds__(kFftThreads, 1) void k_fft_rows(const typename Cplx<R>::T *__restrict__ g,
                                                           typename Cplx<R>::T *__restrict__ out,
                                                           FftPass P,
                                                           const typename Cplx<R>::T *__restrict__ tw,
@@ -226,9 +234,23 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_rows(const typename Cplx<R>
     const int ch = blockIdx.x / P.tiles, t = blockIdx.x % P.tiles;
     const int N = P.N, M = P.M;
     const C2 *src = g + (long long)ch * P.nf + (long long)t * M * P.N2;
-    for (int idx = threadIdx.x; idx < M * N; idx += kFftThreads) {
-        int n = idx & (N - 1), c = idx >> P.lgN;     // N = N2, a power of 2
-        s[fft_lds_idx(n, c, M)] = src[(long long)c * P.N2 + n];
+    constexpr int EPT = kFftMaxElems / kFftThreads, HALF = EPT / 2;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        C2 v[HALF];
+#pragma unroll
+        for (int r = 0; r < HALF; ++r) {
+            int idx = threadIdx.x + (h * HALF + r) * kFftThreads;
+            int n = idx & (N - 1), c = idx >> P.lgN;     // N = N2, a power of 2
+            v[r].x = (R)0;
+            v[r].y = (R)0;
+            if (idx < M * N) v[r] = src[(long long)c * P.N2 + n];
+        }
+#pragma unroll
+        for (int r = 0; r < HALF; ++r) {
+            int idx = threadIdx.x + (h * HALF + r) * kFftThreads;
+            if (idx < M * N) s[fft_lds_idx(idx & (N - 1), idx >> P.lgN, M)] = v[r];
+        }
     }
     __syncthreads();
     fft_lds<R>(s, P, tw + P.w_off);

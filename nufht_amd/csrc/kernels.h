@@ -228,15 +228,23 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
                                              const DirItem *__restrict__ items,
                                              const double *__restrict__ jtab)
 {
+    constexpr int LDC = kDirBM + 1;                 // padded row: no bank conflicts
     __shared__ double s_r[kDirTJ];
-    __shared__ double s_c[kDirTJ * kDirBM];
+    __shared__ double s_c[kDirTJ * LDC];
     DirItem it = items[blockIdx.x];
     const TfDesc &T = tfs[it.tf];
     int nb = min(kDirBM, T.B - it.col0);
-    for (int q = threadIdx.x; q < it.nj; q += 256) s_r[q] = T.r[it.j0 + q];
-    for (int q = threadIdx.x; q < it.nj * kDirBM; q += 256) {
-        int j = q % it.nj, t = q / it.nj;
-        s_c[j * kDirBM + t] = t < nb ? T.cs[(long long)(it.col0 + t) * T.ld_cs + it.j0 + j] : 0.0;
+    {
+        // one source per thread (nj <= kDirTJ = blockDim), its columns loaded together
+        const int j = threadIdx.x;
+        const bool ok = j < it.nj;
+        double cv[kDirBM];
+#pragma unroll
+        for (int t = 0; t < kDirBM; ++t)
+            cv[t] = (ok && t < nb) ? T.cs[(long long)(it.col0 + t) * T.ld_cs + it.j0 + j] : 0.0;
+        s_r[j] = ok ? T.r[it.j0 + j] : 0.0;
+#pragma unroll
+        for (int t = 0; t < kDirBM; ++t) s_c[j * LDC + t] = cv[t];
     }
     __syncthreads();
     int tpr = it.tpr;
@@ -259,7 +267,7 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
                 J = besselj_wr(T.order, T.half, w, s_r[j]);
             }
 #pragma unroll
-            for (int t = 0; t < kDirBM; ++t) acc[t] = fma(J, s_c[j * kDirBM + t], acc[t]);
+            for (int t = 0; t < kDirBM; ++t) acc[t] = fma(J, s_c[j * LDC + t], acc[t]);
         }
     }
     for (int off = tpr >> 1; off > 0; off >>= 1) {
@@ -767,21 +775,31 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
                 }
             }
         } else {
-            for (int idx = tid - 128; idx < kSQ * 16; idx += 128) {
+            // B operand: 8 (source, column) entries per thread, loads issued 4 at a time
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+            double cv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                int idx = tid - 128 + 128 * (4 * h + u);
                 int q = idx & (kSQ - 1), j = idx >> 6;
-                int tt = j % tb, e2 = j / tb;
+                int tt = j % tb;
                 int tc = it.col0 + tt;
-                R v = (R)0;
-                if (q < cn && tc < T.B) {
-                    double c = T.cs[(long long)tc * T.ld_cs + c0 + q];
-                    if (e2 > 0) {
-                        double ri = bx.rho / T.r[c0 + q], rp = ri;
-                        for (int e = 1; e < e2; ++e) rp *= ri;
-                        c *= rp;
-                    }
-                    v = (R)c;
+                cv[u] = (q < cn && tc < T.B) ? T.cs[(long long)tc * T.ld_cs + c0 + q] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                int idx = tid - 128 + 128 * (4 * h + u);
+                int q = idx & (kSQ - 1), j = idx >> 6;
+                int e2 = j / tb;
+                double c = cv[u];
+                if (e2 > 0 && q < cn) {
+                    double ri = bx.rho / T.r[c0 + q], rp = ri;
+                    for (int e = 1; e < e2; ++e) rp *= ri;
+                    c *= rp;
                 }
-                L.bop[q][j] = v;
+                L.bop[q][j] = (R)c;
+            }
             }
         }
         __syncthreads();
@@ -1045,13 +1063,27 @@ __global__ __launch_bounds__(256) void k_asy_interp(const TfDesc *__restrict__ t
     if (use_lds) {
         for (int e0 = 0; e0 < E; e0 += ge) {
             int ne = min(ge, E - e0);
-            for (int q = threadIdx.x; q < ne * tb * pw; q += 256) {
-                int pos = q % pw, row = q / pw;
-                int ee = row / tb, tt = row % tb;
-                int tc = it.col0 + tt;
-                int m = pw0 + pos;
-                m = m < 0 ? m + bx.Nf : m;
-                if (tc < T.B) s_g[q] = grid[bx.goff + (long long)((e0 + ee) * T.B + tc) * bx.Nf + m];
+            // the window rows of ne exponents, 8 loads per thread in flight at a time
+            for (int q0 = 0; q0 < ne * tb * pw; q0 += 256 * 8) {
+                C2 v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    int q = q0 + threadIdx.x + 256 * u;
+                    int pos = q % pw, row = q / pw;
+                    int ee = row / tb, tt = row % tb;
+                    int tc = it.col0 + tt;
+                    int m = pw0 + pos;
+                    m = m < 0 ? m + bx.Nf : m;
+                    v[u].x = (R)0;
+                    v[u].y = (R)0;
+                    if (q < ne * tb * pw && tc < T.B)
+                        v[u] = grid[bx.goff + (long long)((e0 + ee) * T.B + tc) * bx.Nf + m];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    int q = q0 + threadIdx.x + 256 * u;
+                    if (q < ne * tb * pw) s_g[q] = v[u];
+                }
             }
             __syncthreads();
             if (active) {
