@@ -10,8 +10,8 @@
             --copy REGEX --copy-bytes N
         HBM bytes per dispatch of NAME: the average FETCH_SIZE and WRITE_SIZE
         (KiB per dispatch, summed over the counter's instances) of NAME, each
-        scaled by (N bytes) / (the same counter's reading for the copy kernel
-        matching REGEX, which reads N bytes and writes N bytes)
+        scaled by (N bytes) / (the same counter's largest reading for the
+        streaming kernel matching REGEX, which reads N bytes and writes N bytes)
 
 Used to produce the summaries kept with the sources; reads only the
 databases it is given.
@@ -79,11 +79,16 @@ def counters_table(dbs: List[str], match: str = "") -> str:
     return "\n".join(lines)
 
 
-def _avg(vals, kernel_rx: str, counter: str):
+def _readings(vals, kernel_rx: str, counter: str):
     rx = re.compile(kernel_rx)
     per = [v for kn, cs in vals.items() if rx.search(kn) for v in cs.get(counter, {}).values()]
     if not per:
         raise SystemExit(f"no {counter} readings for kernels matching {kernel_rx!r}")
+    return per
+
+
+def _avg(vals, kernel_rx: str, counter: str):
+    per = _readings(vals, kernel_rx, counter)
     return sum(per) / len(per), len(per)
 
 
@@ -92,8 +97,10 @@ def traffic(fetch_db: str, write_db: str, kernel: str, copy_rx: str, copy_bytes:
     kx = "^" + re.escape(kernel) + r"(\b|$|<)"
     f_k, nf = _avg(fv, kx, "FETCH_SIZE")
     w_k, nw = _avg(wv, kx, "WRITE_SIZE")
-    f_c, _ = _avg(fv, copy_rx, "FETCH_SIZE")
-    w_c, _ = _avg(wv, copy_rx, "WRITE_SIZE")
+    # the streaming dispatches read and write copy_bytes; other matches (a fill
+    # of the source buffer) read nothing, so the largest reading is the one
+    f_c = max(_readings(fv, copy_rx, "FETCH_SIZE"))
+    w_c = max(_readings(wv, copy_rx, "WRITE_SIZE"))
     sf = copy_bytes / (f_c * 1024.0)
     sw = copy_bytes / (w_c * 1024.0)
     fetch_b, write_b = f_k * 1024.0 * sf, w_k * 1024.0 * sw
