@@ -584,7 +584,16 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         for (auto &b : dirb) {
             int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
             if (ni <= 0 || nj <= 0) continue;
-            int tpr = ni >= 64 ? 4 : (ni >= 16 ? 16 : 64);
+            // lanes per row: rows with few sources get one lane each (no
+            // reduction); blocks of few rows cover all of them at once
+            int tpr;
+            if (ni >= 64) {
+                tpr = nj <= 32 ? 1 : (nj <= 64 ? 2 : 4);
+            } else {
+                int p2 = 1;
+                while (p2 < ni) p2 <<= 1;
+                tpr = std::min(64, std::max(4, 256 / p2));
+            }
             int rows = 256 / tpr;
             for (int c0 = 0; c0 < B; c0 += kDirBM)
                 for (int i = 0; i < ni; i += rows)
@@ -712,9 +721,13 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 lmin = std::min(lmin, l);
                 lmax = std::max(lmax, l);
             }
+            // a source at x has taps at l = lf .. lf + kw, lf = ceil((x - a1) / h):
+            // it reaches [lmin, lmax] iff (lmin - kw - 1) h + a1 < x <= lmax h + a1;
+            // a quarter cell of margin on each side absorbs rounding (a source the
+            // device places one tap differently has a kernel value ~e^-beta there)
             int2 jr = make_int2(0, 0);
             if (lmin <= lmax) {
-                double xlo = A.C + (lmin - 1) * A.h - A.a1, xhi = A.C + (lmax + 1) * A.h + A.a1;
+                double xlo = A.C + (lmin - T.kw - 1.25) * A.h + A.a1, xhi = A.C + (lmax + 0.25) * A.h + A.a1;
                 jr.x = A.j0 + (int)(std::lower_bound(rb, re, xlo) - rb);
                 jr.y = A.j0 + (int)(std::upper_bound(rb, re, xhi) - rb);
             }
