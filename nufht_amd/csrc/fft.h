@@ -116,10 +116,11 @@ template <typename R> struct Dft<R, 8> {
     }
 };
 
-// LDS position of element n of sequence c: rows of M, one pad element every
-// 8 rows, so that the stride-R writes of a first stage spread over banks.
-__device__ __forceinline__ int fft_lds_idx(int n, int c, int M) { return n * M + c + (n >> 3); }
-__host__ __device__ inline size_t fft_lds_elems(int N, int M) { return (size_t)N * M + (N >> 3) + M; }
+// LDS position of element n of sequence c: rows of M + 1 and one more pad
+// element every 8 rows, so that consecutive n (the row pass's loads) and the
+// stride-R writes of a first stage both spread over the banks.
+__device__ __forceinline__ int fft_lds_idx(int n, int c, int M) { return n * (M + 1) + c + (n >> 3); }
+__host__ __device__ inline size_t fft_lds_elems(int N, int M) { return (size_t)N * (M + 1) + (N >> 3) + M; }
 
 // One Stockham stage of radix RAD over M sequences of length N held in LDS
 // at fft_lds_idx(n, c); Ns = product of the earlier radices.
