@@ -184,7 +184,9 @@ static void gauss_legendre(int q, std::vector<double> &x, std::vector<double> &w
 
 static void es_params(double tol, int mixed, int *kw, double *beta)
 {
-    int w = (int)std::ceil(std::log10(1.0 / tol)) + (mixed ? 1 : 2);
+    // the FP32 grids of mixed mode keep the FP64 width: their rounding
+    // (~1e-7 relative) then stays below the tolerances they serve (>= 1e-6)
+    int w = (int)std::ceil(std::log10(1.0 / tol)) + 2;
     w = std::max(4, std::min(16, w));
     if (mixed) w = std::min(w, 9);
     *kw = w;

@@ -70,6 +70,14 @@ def lib():
     return _LIB
 
 
+def _threads() -> int:
+    """Worker threads for the direct sums: OMP_NUM_THREADS when set, else at
+    most 16 (the host share of one GPU), never more than the cores."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = int(env) if env and env.isdigit() else 16
+    return max(1, min(n, os.cpu_count() or 1))
+
+
 def _d(a):
     return a.ctypes.data_as(_DP)
 
@@ -102,7 +110,7 @@ def direct(nu, r, c, w, nthreads=None) -> np.ndarray:
     n = w.shape[0]
     g = np.zeros((n, B), dtype=np.float64, order="F")
     L.orc_direct(float(nu), _d(r), m, _d(cf), m, B, _d(w), n, _d(g), n,
-                 nthreads or os.cpu_count() or 1)
+                 nthreads or _threads())
     return np.ascontiguousarray(g)
 
 
@@ -118,7 +126,7 @@ def direct_rows(nu, r, c, w, rows, nthreads=None) -> np.ndarray:
     out = np.zeros((rows.shape[0], B), dtype=np.float64, order="F")
     L.orc_direct_rows(float(nu), _d(r), m, _d(cf), m, B, _d(w),
                       rows.ctypes.data_as(_IP), rows.shape[0], _d(out),
-                      nthreads or os.cpu_count() or 1)
+                      nthreads or _threads())
     return np.ascontiguousarray(out)
 
 
