@@ -258,16 +258,12 @@ static double g_dense_cell = [] {
 }();
 
 // FFT lengths restricted to 2^a and 3*2^a (prebuilt rocFFT kernels)
-static int g_fft_pow2 = [] {
-    const char *e = getenv("NUFHT_AMD_FFT_POW2");
-    return e ? atoi(e) : 0;
-}();
 static int fft_length(long long need)
 {
     long long best = 1LL << 40;
     for (long long p = 8; p < (1LL << 34); p <<= 1) {
         if (p >= need) { best = std::min(best, p); break; }
-        if (!g_fft_pow2 && 3 * p >= need) best = std::min(best, 3 * p);
+        if (3 * p >= need) best = std::min(best, 3 * p);
     }
     return (int)best;
 }
@@ -991,8 +987,11 @@ static void launch_asy(Program *P, hipStream_t st)
     }
     if (!P->aimf.empty()) {
         StageTimer tm(ST_INTERP_MF, st);
-        hipLaunchKernelGGL((k_asy_interp_mfma<R, EM>), dim3((unsigned)P->aimf.size()), dim3(64),
-                           0, st, P->d_tfs, P->d_abox, P->d_aimf, (const G *)P->d_grid2, P->d_ft);
+        // FP64 products for both grid precisions: the FP32 matrix-core form of
+        // this kernel measured 1.7x slower on gfx950
+        hipLaunchKernelGGL((k_asy_interp_mfma<R, double, EM>), dim3((unsigned)P->aimf.size()),
+                           dim3(64), 0, st, P->d_tfs, P->d_abox, P->d_aimf,
+                           (const G *)P->d_grid2, P->d_ft);
     }
 }
 

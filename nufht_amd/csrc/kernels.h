@@ -873,20 +873,24 @@ __global__ __launch_bounds__(256) void k_asy_spread_mfma(const TfDesc *__restric
 //   D[i][(t, e2)] = sum_m phi(s_i - m Delta) grid[e][t][m],  e = g nsub + e2
 // with the phi matrix computed once per tile; the FP64 epilogue applies
 // e^{i(phi0 + w C)} h Delta / psihat(s) and the Hankel weights.
-// (GR: the grid's precision; the products run in FP64 either way, which on
-// this hardware is faster than the FP32 form of this kernel)
-template <typename GR, int EMAX>
+// (GR: the grid's precision; CR: the products' precision.)  The Hankel
+// combination is folded into a running complex sum per (target, column):
+// S = sum_e A'_e i^(e mod 2) G_e with A'_e = +-a_e (rho w)^-e, and the output
+// is b0 Re(f S) with f = e^{i(phi0 + w C)} h Delta / psihat(s): the real part
+// for even e and minus the imaginary part for odd e, as the reference's
+// cos / sin terms (src/expansions.c:523-531, :549-557).
+template <typename GR, typename CR, int EMAX>
 __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict__ tfs,
                                                         const AsyBoxD *__restrict__ boxes,
                                                         const AsyItem *__restrict__ items,
                                                         const typename Cplx<GR>::T *__restrict__ grid,
                                                         const double *__restrict__ ft)
 {
-    typedef double R;
+    typedef CR R;
     typedef typename Cplx<GR>::T C2;
     typedef typename Mfma<R>::V V;
     __shared__ double s_s[16], s_fr[16], s_fi[16], s_b0[16];
-    __shared__ double s_pw[16][EMAX];
+    __shared__ double s_P[16][EMAX];
     __shared__ int s_m0[16];
     AsyItem it = items[blockIdx.x];
     AsyBoxD bx = boxes[it.box];
@@ -903,7 +907,7 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
         s_m0[ti] = (int)ceil((s - bx.a2) / bx.delta);
         double nu = (double)T.order + 0.5 * (double)T.half;
         double sp, cp;
-        sincos(w * bx.C, &sp, &cp);
+        sincos_cw(w * bx.C, &sp, &cp);
         int q = (int)lround(2.0 * nu + 1.0) & 7;
         double c0 = cos_qpi4(q), s0 = -cos_qpi4((q + 6) & 7);
         double amp = bx.h * bx.delta / (bx.a1 * ft_eval(ft + T.ft_off, s * bx.a1, T.kmax));
@@ -912,7 +916,7 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
         s_b0[ti] = ti < it.count ? T.sign * sqrt(2.0 / kPi) / sqrt(w) : 0.0;
         double rw = 1.0 / (bx.rho * w), pw = 1.0;
         for (int e = 0; e < EMAX; ++e) {
-            s_pw[ti][e] = pw;
+            s_P[ti][e] = e < E ? ((((e >> 1) & 1) ? -T.a[e] : T.a[e]) * pw) : 0.0;
             pw *= rw;
         }
     }
@@ -931,37 +935,51 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
     const int jb = lane & 15, tbk = jb % tb, e2b = jb / tb;
     const int tcol = it.col0 + tbk;
     const bool colok = tcol < T.B;
-    double out[4] = {0.0, 0.0, 0.0, 0.0};
+    int moff[4];
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        int m = mw0 + 4 * st + ka;
+        moff[st] = m < 0 ? m + bx.Nf : (m >= bx.Nf ? m - bx.Nf : m);
+    }
+    double Sr[4] = {0.0, 0.0, 0.0, 0.0}, Si[4] = {0.0, 0.0, 0.0, 0.0};
     const int G = (E + nsub - 1) / nsub;
     for (int g = 0; g < G; ++g) {
-        int eb = g * nsub + e2b;
-        bool eok = colok && eb < E;
-        const C2 *row = grid + bx.goff + (long long)(eb * T.B + tcol) * bx.Nf;
-        V dr, di;
+        const int eb = g * nsub + e2b;
+        const bool eok = colok && eb < E;
+        // the step's 4 grid values: loads at valid addresses, issued together
+        const C2 *row = grid + bx.goff + (long long)((eok ? eb : 0) * T.B + (colok ? tcol : 0)) * bx.Nf;
+        C2 b[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) { dr[v] = (R)0; di[v] = (R)0; }
+        for (int st = 0; st < 4; ++st) b[st] = row[moff[st]];
+        V dr0, di0, dr1, di1;
 #pragma unroll
-        for (int st = 0; st < 4; ++st) {
-            if (st < nsteps) {
-                int m = mw0 + 4 * st + ka;
-                m = m < 0 ? m + bx.Nf : (m >= bx.Nf ? m - bx.Nf : m);
-                C2 b;
-                b.x = (GR)0;
-                b.y = (GR)0;
-                if (eok) b = row[m];
-                dr = Mfma<R>::mma(phiA[st], (R)b.x, dr);
-                di = Mfma<R>::mma(phiA[st], (R)b.y, di);
-            }
-        }
+        for (int v = 0; v < 4; ++v) { dr0[v] = (R)0; di0[v] = (R)0; dr1[v] = (R)0; di1[v] = (R)0; }
+        const R zero = (R)0;
+        dr0 = Mfma<R>::mma(phiA[0], eok ? (R)b[0].x : zero, dr0);
+        di0 = Mfma<R>::mma(phiA[0], eok ? (R)b[0].y : zero, di0);
+        dr1 = Mfma<R>::mma(phiA[1], eok ? (R)b[1].x : zero, dr1);
+        di1 = Mfma<R>::mma(phiA[1], eok ? (R)b[1].y : zero, di1);
+        dr0 = Mfma<R>::mma(phiA[2], eok ? (R)b[2].x : zero, dr0);
+        di0 = Mfma<R>::mma(phiA[2], eok ? (R)b[2].y : zero, di0);
+        dr1 = Mfma<R>::mma(phiA[3], eok ? (R)b[3].x : zero, dr1);
+        di1 = Mfma<R>::mma(phiA[3], eok ? (R)b[3].y : zero, di1);
+        const bool odd = eb & 1;
+        const int ec = eok ? eb : 0;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             int i = Mfma<R>::row(lane, v);
-            if (!eok || i >= it.count) continue;
-            double gr = (double)dr[v], gi = (double)di[v];
-            double vr = s_fr[i] * gr - s_fi[i] * gi, vi = s_fr[i] * gi + s_fi[i] * gr;
-            double A = (((eb >> 1) & 1) ? -T.a[eb] : T.a[eb]) * s_b0[i] * s_pw[i][eb];
-            out[v] = fma(A, (eb & 1) ? -vi : vr, out[v]);
+            double p = s_P[i][ec];
+            double gr = (double)dr0[v] + (double)dr1[v], gi = (double)di0[v] + (double)di1[v];
+            // S += p i^(e mod 2) G
+            Sr[v] = fma(p, odd ? -gi : gr, Sr[v]);
+            Si[v] = fma(p, odd ? gr : gi, Si[v]);
         }
+    }
+    double out[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        int i = Mfma<R>::row(lane, v);
+        out[v] = s_b0[i] * (s_fr[i] * Sr[v] - s_fi[i] * Si[v]);
     }
     // fold the exponent sub-split (lanes j and j + tb * e2 share a column)
     for (int off = tb; off < 16; off <<= 1)
