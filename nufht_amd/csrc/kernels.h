@@ -1077,7 +1077,7 @@ __global__ __launch_bounds__(256) void k_asy_interp(const TfDesc *__restrict__ t
     R phi[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) phi[k] = (active && k < kw) ? s_phi[ti][k] : (R)0;
-    double out = 0.0;
+    double Sr = 0.0, Si = 0.0;
     if (use_lds) {
         for (int e0 = 0; e0 < E; e0 += ge) {
             int ne = min(ge, E - e0);
@@ -1117,9 +1117,10 @@ __global__ __launch_bounds__(256) void k_asy_interp(const TfDesc *__restrict__ t
                         }
                     }
                     int e = e0 + ee;
-                    double vr = fr * (double)gr - fi * (double)gi, vi = fr * (double)gi + fi * (double)gr;
                     double A = (((e >> 1) & 1) ? -T.a[e] : T.a[e]) * base;
-                    out = fma(A, (e & 1) ? -vi : vr, out);
+                    // S += A i^(e mod 2) G; the output is Re(f S)
+                    Sr = fma(A, (e & 1) ? -(double)gi : (double)gr, Sr);
+                    Si = fma(A, (e & 1) ? (double)gr : (double)gi, Si);
                     base *= rw;
                 }
             }
@@ -1139,13 +1140,14 @@ __global__ __launch_bounds__(256) void k_asy_interp(const TfDesc *__restrict__ t
                     gi = fma(phi[k], v.y, gi);
                 }
             }
-            double vr = fr * (double)gr - fi * (double)gi, vi = fr * (double)gi + fi * (double)gr;
             double A = (((e >> 1) & 1) ? -T.a[e] : T.a[e]) * base;
-            out = fma(A, (e & 1) ? -vi : vr, out);
+            Sr = fma(A, (e & 1) ? -(double)gi : (double)gr, Sr);
+            Si = fma(A, (e & 1) ? (double)gr : (double)gi, Si);
             base *= rw;
         }
     }
-    if (active) atomicAdd(&T.gs[(long long)t * T.ld_gs + bx.i0 + it.start + ti], out);
+    if (active)
+        atomicAdd(&T.gs[(long long)t * T.ld_gs + bx.i0 + it.start + ti], fr * Sr - fi * Si);
 }
 
 // ------------------------------------------------------------------------
