@@ -282,13 +282,14 @@ static void fft_radices(int N, FftPass &p)
     if (k % 3 == 1) p.radix[p.nstage++] = 2;
     if (three) p.radix[p.nstage++] = 3;
 }
-static int fft_pick_m(int N, int other)
+static int fft_pick_m(int N, int other, int maxe)
 {
     int m = 16;
-    while (m > 1 && (m * N > kFftMaxElems || other % m)) m >>= 1;
+    while (m > 1 && (m * N > maxe || other % m)) m >>= 1;
     return m;
 }
-static bool fft_factor(int nf, FftPass &p1, FftPass &p2)
+// maxe: elements per block (fft_max_elems of the grid precision)
+static bool fft_factor(int nf, int maxe, FftPass &p1, FftPass &p2)
 {
     if (nf <= 4096) return false;
     int b3 = nf % 3 == 0, q = b3 ? nf / 3 : nf, a = 0;
@@ -301,7 +302,7 @@ static bool fft_factor(int nf, FftPass &p1, FftPass &p2)
         if (nf % N2) continue;
         int N1 = nf / N2;
         if (N1 < 16 || N1 > 1536) continue;
-        int m1 = fft_pick_m(N1, N2), m2 = fft_pick_m(N2, N1);
+        int m1 = fft_pick_m(N1, N2, maxe), m2 = fft_pick_m(N2, N1, maxe);
         int score = 64 * std::min(m1, m2) + std::max(m1, m2);
         if (score > bestScore) { bestScore = score; bestN2 = N2; }
     }
@@ -313,10 +314,10 @@ static bool fft_factor(int nf, FftPass &p1, FftPass &p2)
         p->nf = nf;
     }
     p1.N = N1;
-    p1.M = fft_pick_m(N1, N2);
+    p1.M = fft_pick_m(N1, N2, maxe);
     p1.tiles = N2 / p1.M;
     p2.N = N2;
-    p2.M = fft_pick_m(N2, N1);
+    p2.M = fft_pick_m(N2, N1, maxe);
     p2.tiles = N1 / p2.M;
     for (FftPass *p : {&p1, &p2}) {
         p->lgM = 0;
@@ -372,8 +373,8 @@ struct Program {
     LocItem *d_lproj = nullptr, *d_leval = nullptr;
     double *d_coef = nullptr;
     long long ncoef = 0;
-    int loc_ch = 256, loc_tpb = 64, loc_stride = 0;
-    size_t loc_lds_p = 0, loc_lds_e = 0;
+    int loc_ch = 1024, loc_tpb = 256, loc_nlt = 1;
+    size_t loc_lds_e = 0;
     std::vector<AsyBoxD> abox;
     AsyBoxD *d_abox = nullptr;
     std::vector<AsyItem> aspread, ainterp, azero, adense, aimf;
@@ -504,9 +505,9 @@ static int effective_prec(const NufhtPlan *plan)
 // program construction
 struct TfInput { const NufhtPlan *plan; int B; };
 
-static void add_chunks_2d(std::vector<LocItem> &v, int box, int count, int chunk, int B)
+static void add_chunks_2d(std::vector<LocItem> &v, int box, int count, int chunk, int B, int cols)
 {
-    for (int c0 = 0; c0 < B; c0 += kLocBM)
+    for (int c0 = 0; c0 < B; c0 += cols)
         for (int s = 0; s < count; s += chunk) v.push_back({box, s, std::min(chunk, count - s), c0});
 }
 
@@ -523,7 +524,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     memset(&P->stats, 0, sizeof(P->stats));
     std::vector<double> ft_all, cheb_all;
     std::map<std::pair<int, double>, int> ft_off;
-    int kloc_max = 0, maxo_max = 0, e_max = 0;
+    int kloc_max = 0, slots_max = 0, e_max = 0;
     struct AsyPending { AsyBoxD b; int tb; };
     std::vector<AsyPending> apend;
     for (size_t k = 0; k < in.size(); ++k) {
@@ -603,8 +604,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         }
         if (!locb.empty()) {
             kloc_max = std::max(kloc_max, T.K_loc);
-            int maxo = T.half ? T.K_loc : (T.order >> 1) + T.K_loc + (T.order & 1);
-            maxo_max = std::max(maxo_max, maxo);
+            slots_max = std::max(slots_max, loc_slots(T.order, T.half, T.K_loc));
             if (T.half) {
                 int K = T.K_loc, N = 2 * K + 2, p = T.order & 1;
                 T.cheb_off = (int)cheb_all.size();
@@ -674,7 +674,8 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         FftClass c;
         c.nf = kv.first;
         c.off = off;
-        c.own = fft_factor(c.nf, c.p1, c.p2);
+        c.own = fft_factor(c.nf, mixed ? fft_max_elems<float>() : fft_max_elems<double>(), c.p1,
+                           c.p2);
         own_nf[c.nf] = c.own;
         c.p1.chinfo_off = c.p2.chinfo_off = (int)P->fft_chinfo.size();
         for (int q : kv.second) {
@@ -813,19 +814,32 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     }
     P->stats.fft_classes = (long long)P->fft.size();
     P->emax = e_max <= 6 ? 6 : e_max <= 10 ? 10 : e_max <= 14 ? 14 : 22;
-    // local-expansion launch geometry (LDS budget 64 KiB)
+    // local-expansion launch geometry: projection items of loc_ch sources x
+    // 16 columns with ceil((K+1)/16) row tiles; evaluation items of loc_tpb
+    // targets x all columns, the per-target scratch within 48 KiB of LDS
     if (!P->lbox.empty()) {
-        int kp1 = kloc_max + 1;
-        P->loc_ch = std::max(16, std::min(256, 7680 / (kp1 + kLocBM)));
-        P->loc_lds_p = (size_t)(kp1 * P->loc_ch + P->loc_ch * kLocBM) * sizeof(double);
-        P->loc_stride = kp1 + std::max(maxo_max + 1, kp1);
-        int tpb = (int)((60000 / sizeof(double) - (size_t)kp1 * kLocBM) / P->loc_stride);
-        P->loc_tpb = std::max(1, std::min(64, tpb));
-        P->loc_lds_e = (size_t)(P->loc_tpb * P->loc_stride + kp1 * kLocBM) * sizeof(double);
+        P->loc_nlt = (kloc_max + 16) / 16;
+        if (P->loc_nlt > kLocMaxNlt) {
+            nufht_set_error("local expansion with K_loc = %d exceeds the supported %d", kloc_max,
+                            16 * kLocMaxNlt - 1);
+            return -540;
+        }
+        P->loc_tpb = 256;
+        while (P->loc_tpb > 64 && (size_t)slots_max * P->loc_tpb * sizeof(double) > 48 * 1024)
+            P->loc_tpb >>= 1;
+        P->loc_lds_e = (size_t)slots_max * P->loc_tpb * sizeof(double);
+        if (P->loc_lds_e > 160 * 1024) {
+            nufht_set_error("local expansion scratch of %d words per target exceeds LDS", slots_max);
+            return -540;
+        }
+        if (P->loc_lds_e > 64 * 1024)
+            HIP_TRY(hipFuncSetAttribute((const void *)k_loc_eval,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)P->loc_lds_e));
         for (size_t b = 0; b < P->lbox.size(); ++b) {
             const TfDesc &T = P->tfs[P->lbox[b].tf];
-            add_chunks_2d(P->lproj, (int)b, P->lbox[b].nj, P->loc_ch, T.B);
-            add_chunks_2d(P->leval, (int)b, P->lbox[b].ni, P->loc_tpb, T.B);
+            add_chunks_2d(P->lproj, (int)b, P->lbox[b].nj, P->loc_ch, T.B, kLocBM);
+            add_chunks_2d(P->leval, (int)b, P->lbox[b].ni, P->loc_tpb, T.B, T.B);
         }
     }
     if (!P->tfs.empty()) {
@@ -1006,6 +1020,24 @@ static void launch_asy_e(Program *P, hipStream_t st)
     }
 }
 
+template <int NLT> static void launch_loc_proj_n(Program *P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_loc_proj<NLT>, dim3((unsigned)P->lproj.size()), dim3(256),
+                       LocProjLds<NLT>::bytes, st, P->d_tfs, P->d_lbox, P->d_lproj, P->d_coef);
+}
+
+static void launch_loc_proj(Program *P, hipStream_t st)
+{
+    switch (P->loc_nlt) {
+    case 1: launch_loc_proj_n<1>(P, st); break;
+    case 2: launch_loc_proj_n<2>(P, st); break;
+    case 3: launch_loc_proj_n<3>(P, st); break;
+    case 4: launch_loc_proj_n<4>(P, st); break;
+    case 5: launch_loc_proj_n<5>(P, st); break;
+    default: launch_loc_proj_n<6>(P, st); break;
+    }
+}
+
 static int run_program(Program *P, hipStream_t st)
 {
     if (P->desc_dirty) {
@@ -1024,15 +1056,13 @@ static int run_program(Program *P, hipStream_t st)
     if (!P->lproj.empty()) {
         {
             StageTimer tm(ST_LOCP, st);
-            hipLaunchKernelGGL(k_loc_proj, dim3((unsigned)P->lproj.size()), dim3(256),
-                               P->loc_lds_p, st, P->d_tfs, P->d_lbox, P->d_lproj, P->d_coef,
-                               P->loc_ch);
+            launch_loc_proj(P, st);
         }
         {
             StageTimer tm(ST_LOCE, st);
             hipLaunchKernelGGL(k_loc_eval, dim3((unsigned)P->leval.size()), dim3(P->loc_tpb),
                                P->loc_lds_e, st, P->d_tfs, P->d_lbox, P->d_leval, P->d_coef,
-                               P->d_cheb, P->loc_stride);
+                               P->d_cheb);
         }
     }
     if (!P->dir.empty()) {

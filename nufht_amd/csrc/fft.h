@@ -19,7 +19,10 @@
 namespace nufht_dev {
 
 constexpr int kFftThreads = 512;
-constexpr int kFftMaxElems = 8192;    // M * N per block
+// M * N per block: 64 KiB of data, so that two blocks share a CU and one
+// block's loads overlap the other's LDS passes
+constexpr int kFftTileBytes = 65536;
+template <typename R> constexpr int fft_max_elems() { return kFftTileBytes / (2 * (int)sizeof(R)); }
 constexpr int kFftMaxStages = 8;
 
 struct FftPass {
@@ -130,7 +133,7 @@ __device__ __forceinline__ void fft_stage(typename Cplx<R>::T *s, int N, int M, 
                                           const typename Cplx<R>::T *__restrict__ W)
 {
     typedef typename Cplx<R>::T C2;
-    constexpr int MAXB = (kFftMaxElems / RAD + kFftThreads - 1) / kFftThreads;
+    constexpr int MAXB = (fft_max_elems<R>() / RAD + kFftThreads - 1) / kFftThreads;
     const int Ns = 1 << lgNs;
     const int nb = M * N / RAD, span = N / RAD;
     const int wstep = N / (Ns * RAD);
@@ -180,7 +183,7 @@ __device__ __forceinline__ void fft_lds(typename Cplx<R>::T *s, const FftPass &P
 
 // Pass 1: column FFTs of length N1 (stride N2) in place, times w^{k1 n2}.
 template <typename R>
-__global__ __launch_bounds__(kFftThreads, 1) void k_fft_cols(typename Cplx<R>::T *__restrict__ g,
+__global__ __launch_bounds__(kFftThreads, 2) void k_fft_cols(typename Cplx<R>::T *__restrict__ g,
                                                           FftPass P,
                                                           const typename Cplx<R>::T *__restrict__ tw,
                                                           const int4 *__restrict__ chinfo)
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(kFftThreads, 1) void k_fft_cols(typename Cplx<R>::T
     const int nf = (int)P.nf;
     C2 *base = g + (long long)ch * P.nf + (long long)t * M;
     // all of a thread's loads in flight at once, then into LDS
-    constexpr int EPT = kFftMaxElems / kFftThreads;
+    constexpr int EPT = fft_max_elems<R>() / kFftThreads;
     C2 v[EPT];
 #pragma unroll
     for (int r = 0; r < EPT; ++r) {
@@ -223,7 +226,7 @@ __global__ __launch_bounds__(kFftThreads, 1) void k_fft_cols(typename Cplx<R>::T
 
 // Pass 2: row FFTs of length N2, written to natural order in out.
 template <typename R>
-__global__ __launch_bounds__(kFftThreads, 1) void k_fft_rows(const typename Cplx<R>::T *__restrict__ g,
+__global__ __launch_bounds__(kFftThreads, 2) void k_fft_rows(const typename Cplx<R>::T *__restrict__ g,
                                                           typename Cplx<R>::T *__restrict__ out,
                                                           FftPass P,
                                                           const typename Cplx<R>::T *__restrict__ tw,
@@ -235,7 +238,7 @@ __global__ __launch_bounds__(kFftThreads, 1) void k_fft_rows(const typename Cplx
     const int ch = blockIdx.x / P.tiles, t = blockIdx.x % P.tiles;
     const int N = P.N, M = P.M;
     const C2 *src = g + (long long)ch * P.nf + (long long)t * M * P.N2;
-    constexpr int EPT = kFftMaxElems / kFftThreads, HALF = EPT / 2;
+    constexpr int EPT = fft_max_elems<R>() / kFftThreads, HALF = EPT / 2;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         C2 v[HALF];

@@ -281,150 +281,6 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
 }
 
 // ------------------------------------------------------------------------
-// Local expansion, projection half: coef[l][t] += sum_j B_l(x_j) c_j,t with
-// x_j = r_j / r_end.  Integer orders follow the reference's Wimp expansion
-// (src/expansions.c:242-273): even nu -> eps_l T_2l(x), odd nu -> 2 T_2l+1(x).
-// Half-integer orders (Chebyshev interpolation of j_n, this build) use
-// T_2l+p(x), p = n mod 2, with weights c_j sqrt(r_j).
-// Chebyshev polynomials come from the three-term recurrence.
-constexpr int kLocBM = 16;
-
-__global__ __launch_bounds__(256) void k_loc_proj(const TfDesc *__restrict__ tfs,
-                                                  const LocBoxD *__restrict__ boxes,
-                                                  const LocItem *__restrict__ items,
-                                                  double *__restrict__ coef, int ch)
-{
-    extern __shared__ double sm[];
-    LocItem it = items[blockIdx.x];
-    LocBoxD bx = boxes[it.box];
-    const TfDesc &T = tfs[bx.tf];
-    int K = T.K_loc;
-    double *sT = sm;                       // [(K+1)][ch]
-    double *sC = sm + (size_t)(K + 1) * ch; // [ch][kLocBM]
-    int nb = min(kLocBM, T.B - it.col0);
-    int par = T.half ? (T.order & 1) : (T.order & 1);
-    for (int q = threadIdx.x; q < it.count; q += blockDim.x) {
-        int j = bx.j0 + it.start + q;
-        double rj = T.r[j];
-        double x = rj / bx.r_end;
-        x = fmin(1.0, fmax(-1.0, x));
-        double tm1 = 1.0, t0 = x;          // T_0, T_1
-        // basis index l <-> Chebyshev degree 2l + par
-        if (par == 0) sT[q] = 1.0;
-        else sT[q] = 2.0 * x;
-        int deg = 1;
-        for (int l = 1; l <= K; ++l) {
-            int target = 2 * l + par;
-            while (deg < target) {
-                double tn = 2.0 * x * t0 - tm1;
-                tm1 = t0;
-                t0 = tn;
-                deg++;
-            }
-            sT[(size_t)l * ch + q] = 2.0 * t0;
-        }
-        double wgt = T.half ? sqrt(rj) : 1.0;
-        for (int t = 0; t < kLocBM; ++t)
-            sC[q * kLocBM + t] = t < nb ? wgt * T.cs[(long long)(it.col0 + t) * T.ld_cs + j] : 0.0;
-    }
-    __syncthreads();
-    for (int p = threadIdx.x; p < (K + 1) * nb; p += blockDim.x) {
-        int l = p / nb, t = p % nb;
-        double acc = 0.0;
-        const double *tl = sT + (size_t)l * ch;
-        for (int q = 0; q < it.count; ++q) acc = fma(tl[q], sC[q * kLocBM + t], acc);
-        atomicAdd(&coef[bx.coef_off + (long long)l * T.B + it.col0 + t], acc);
-    }
-}
-
-// Local expansion, evaluation half (reference src/expansions.c:275-336):
-// integer orders compute J_0..J_{l0+K+odd}(z), z = w r_end / 2, by one
-// Miller sweep and form b_l = J_{l0+odd+l}(z) J_{|l0-l|}(z) (sign (-1) for
-// l > l0 with l - l0 odd); half-integer orders sample j_n(w r_end x_q) at
-// the K+1 positive Chebyshev nodes and map them to coefficients with the
-// precomputed matrix cheb[].  Then g[i,t] += sum_l b_l coef[l][t].
-__global__ void k_loc_eval(const TfDesc *__restrict__ tfs, const LocBoxD *__restrict__ boxes,
-                           const LocItem *__restrict__ items, const double *__restrict__ coef,
-                           const double *__restrict__ cheb, int stride_j)
-{
-    extern __shared__ double sm[];
-    LocItem it = items[blockIdx.x];
-    LocBoxD bx = boxes[it.box];
-    const TfDesc &T = tfs[bx.tf];
-    int K = T.K_loc;
-    int tid = threadIdx.x, nthr = blockDim.x;
-    double *sJ = sm + (size_t)tid * stride_j;              // per-thread scratch
-    double *sCoef = sm + (size_t)nthr * stride_j;          // [(K+1)][kLocBM]
-    int nb = min(kLocBM, T.B - it.col0);
-    for (int p = tid; p < (K + 1) * kLocBM; p += nthr) {
-        int l = p / kLocBM, t = p % kLocBM;
-        sCoef[p] = t < nb ? coef[bx.coef_off + (long long)l * T.B + it.col0 + t] : 0.0;
-    }
-    __syncthreads();
-    if (tid >= it.count) return;
-    int i = bx.i0 + it.start + tid;
-    double w = T.w[i];
-    double scale = 1.0;
-    double *b = sJ;   // b_l overwrites the front of the scratch once used
-    if (!T.half) {
-        int l0 = T.order >> 1, odd = T.order & 1;
-        int maxo = l0 + K + odd;
-        double z = w * bx.r_end * 0.5;
-        double *J = sJ + (K + 1);
-        if (z == 0.0) {
-            for (int k = 0; k <= maxo; ++k) J[k] = (k == 0) ? 1.0 : 0.0;
-        } else {
-            double m = (double)(maxo > (int)z ? maxo : (int)z);
-            int N = (int)(m + 20.0 + 4.0 * sqrt(m + 1.0) + 6.0 * cbrt(m + 1.0));
-            N += N & 1;
-            double jp1 = 0.0, j = 1e-280, sum = 0.0, tx = 2.0 / z;
-            for (int k = N; k >= 1; --k) {
-                double jm1 = (double)k * tx * j - jp1;
-                jp1 = j;
-                j = jm1;
-                int km1 = k - 1;
-                if (km1 <= maxo) J[km1] = j;
-                if (km1 > 0 && !(km1 & 1)) sum += 2.0 * j;
-                if (fabs(j) > 1e200) {
-                    j *= 1e-200; jp1 *= 1e-200; sum *= 1e-200;
-                    for (int q = km1; q <= maxo; ++q) J[q] *= 1e-200;
-                }
-            }
-            double inv = 1.0 / (sum + j);
-            for (int k = 0; k <= maxo; ++k) J[k] *= inv;
-        }
-        for (int l = 0; l <= K; ++l) {
-            int lo = l0 - l;
-            lo = lo < 0 ? -lo : lo;
-            double v = J[l0 + odd + l] * J[lo];
-            if (l > l0 && ((l - l0) & 1)) v = -v;
-            b[l] = v;
-        }
-    } else {
-        // f_q = j_n(z x_q), x_q = cos(pi (q + 1/2) / (2K + 2)), q = 0..K
-        double z = w * bx.r_end;
-        double *f = sJ + (K + 1);
-        for (int q = 0; q <= K; ++q) {
-            double xq = cos(kPi * ((double)q + 0.5) / (double)(2 * K + 2));
-            double hi = z * xq, lo = fma(z, xq, -hi);
-            f[q] = sph_jn(T.order, hi, lo);
-        }
-        const double *M = cheb + T.cheb_off;
-        for (int l = 0; l <= K; ++l) {
-            double a = 0.0;
-            for (int q = 0; q <= K; ++q) a = fma(M[l * (K + 1) + q], f[q], a);
-            b[l] = a;
-        }
-        scale = sqrt(2.0 * w / kPi);
-    }
-    for (int t = 0; t < nb; ++t) {
-        double acc = 0.0;
-        for (int l = 0; l <= K; ++l) acc = fma(b[l], sCoef[l * kLocBM + t], acc);
-        atomicAdd(&T.gs[(long long)(it.col0 + t) * T.ld_gs + i], T.sign * scale * acc);
-    }
-}
-
-// ------------------------------------------------------------------------
 // Asymptotic regime, reference add_asy_batch (src/expansions.c:483-559):
 //   g[i,t] += sum_e A_e(w_i) Re/Im( e^{i phi} F_e,t(w_i) ),
 //   F_e,t(w) = sum_j c_j,t r_j^{-e-1/2} e^{i w r_j},  e = 2l + half,
@@ -642,6 +498,228 @@ template <> struct Mfma<float> {
     }
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
+
+// ------------------------------------------------------------------------
+// Local expansion, projection half: coef[l][t] += sum_j B_l(x_j) c_j,t with
+// x_j = r_j / r_end.  Integer orders follow the reference's Wimp expansion
+// (src/expansions.c:242-273): even nu -> eps_l T_2l(x), odd nu -> 2 T_2l+1(x).
+// Half-integer orders (Chebyshev interpolation of j_n, this build) use
+// T_2l+p(x), p = n mod 2, with weights c_j sqrt(r_j).
+//
+// The contraction is a (K+1) x count x 16 matrix product per item, run on
+// the FP64 matrix cores: A[l][q] = B_l(x_q) (NLT row tiles of 16), B[q][t] =
+// c_q,t, accumulated in registers over every source of the item; a pass
+// stages kLocSub sources (basis by the three-term recurrence, one source per
+// lane; coefficients loaded column by column, coalesced) in LDS with odd row
+// strides, the 4 waves take interleaved K steps of 4 sources, and the 4
+// partial tiles are summed through LDS before one atomic add per output.
+constexpr int kLocBM = 16;     // columns per item
+constexpr int kLocSub = 64;    // sources staged per pass
+constexpr int kLocMaxNlt = 6;  // row tiles: K_loc <= 95
+
+template <int NLT> struct LocProjLds {
+    static constexpr int LT = 16 * NLT + 1;
+    static constexpr size_t stage = (size_t)kLocSub * (LT + kLocBM + 1);
+    static constexpr size_t red = (size_t)4 * 16 * NLT * (kLocBM + 1);
+    static constexpr size_t bytes = (stage > red ? stage : red) * sizeof(double);
+};
+
+template <int NLT>
+__global__ __launch_bounds__(256) void k_loc_proj(const TfDesc *__restrict__ tfs,
+                                                  const LocBoxD *__restrict__ boxes,
+                                                  const LocItem *__restrict__ items,
+                                                  double *__restrict__ coef)
+{
+    constexpr int LT = LocProjLds<NLT>::LT, LC = kLocBM + 1;
+    extern __shared__ double sm[];
+    double *sT = sm;                          // [kLocSub][LT]
+    double *sC = sm + (size_t)kLocSub * LT;   // [kLocSub][LC]
+    LocItem it = items[blockIdx.x];
+    LocBoxD bx = boxes[it.box];
+    const TfDesc &T = tfs[bx.tf];
+    const int K = T.K_loc, par = T.order & 1;
+    const int nb = min(kLocBM, T.B - it.col0);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    mf_d4 d[NLT];
+#pragma unroll
+    for (int lt = 0; lt < NLT; ++lt)
+        for (int v = 0; v < 4; ++v) d[lt][v] = 0.0;
+    for (int s0 = 0; s0 < it.count; s0 += kLocSub) {
+        const int n = min(kLocSub, it.count - s0);
+        __syncthreads();
+        if (tid < kLocSub) {
+            double *row = sT + (size_t)tid * LT;
+            if (tid < n) {
+                double x = T.r[bx.j0 + it.start + s0 + tid] / bx.r_end;
+                x = fmin(1.0, fmax(-1.0, x));
+                double tm1 = 1.0, t0 = x;   // T_0, T_1
+                row[0] = par ? 2.0 * x : 1.0;
+                int deg = 1;
+                for (int l = 1; l < 16 * NLT; ++l) {
+                    if (l <= K) {
+                        for (int target = 2 * l + par; deg < target; ++deg) {
+                            double tn = 2.0 * x * t0 - tm1;
+                            tm1 = t0;
+                            t0 = tn;
+                        }
+                        row[l] = 2.0 * t0;
+                    } else {
+                        row[l] = 0.0;
+                    }
+                }
+            } else {
+                for (int l = 0; l < 16 * NLT; ++l) row[l] = 0.0;
+            }
+        }
+        // coefficients: lanes run over sources, 256 / kLocSub columns at a time
+        for (int e = tid; e < kLocSub * kLocBM; e += 256) {
+            int q = e % kLocSub, t = e / kLocSub;
+            double c = 0.0;
+            if (q < n && t < nb) {
+                int j = bx.j0 + it.start + s0 + q;
+                c = T.cs[(long long)(it.col0 + t) * T.ld_cs + j];
+                if (T.half) c *= sqrt(T.r[j]);
+            }
+            sC[q * LC + t] = c;
+        }
+        __syncthreads();
+        const int ia = lane & 15, ka = lane >> 4;
+        for (int st = wave; 4 * st < n; st += 4) {
+            const int q = 4 * st + ka;
+            const double bv = sC[q * LC + ia];
+#pragma unroll
+            for (int lt = 0; lt < NLT; ++lt)
+                d[lt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sT[q * LT + 16 * lt + ia], bv, d[lt],
+                                                            0, 0, 0);
+        }
+    }
+    // sum the 4 waves' tiles: red[wave][16 NLT][LC]
+    __syncthreads();
+    double *red = sm;
+#pragma unroll
+    for (int lt = 0; lt < NLT; ++lt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int row = 16 * lt + (lane >> 4) + 4 * v;
+            red[((size_t)wave * 16 * NLT + row) * LC + (lane & 15)] = d[lt][v];
+        }
+    __syncthreads();
+    for (int e = tid; e < 16 * NLT * 16; e += 256) {
+        int row = e >> 4, t = e & 15;
+        if (row > K || t >= nb) continue;
+        double acc = 0.0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) acc += red[((size_t)w * 16 * NLT + row) * LC + t];
+        atomicAdd(&coef[bx.coef_off + (long long)row * T.B + it.col0 + t], acc);
+    }
+}
+
+// Local expansion, evaluation half (reference src/expansions.c:275-336):
+// integer orders compute J_0..J_{l0+K+odd}(z), z = w r_end / 2, by one
+// Miller sweep and form b_l = J_{l0+odd+l}(z) J_{|l0-l|}(z) (sign (-1) for
+// l > l0 with l - l0 odd); half-integer orders sample j_n(w r_end x_q) at
+// the K+1 positive Chebyshev nodes and map them to coefficients with the
+// precomputed matrix cheb[].  Then g[i,t] += sum_l b_l coef[l][t] for every
+// column t.  One thread per target; the per-target scratch lives in LDS as
+// [slot][thread] (slots = loc_slots()), so lanes touch consecutive words;
+// the coefficients are the same for the whole block (one box) and are read
+// as uniform loads, four columns at a time.
+__host__ __device__ inline int loc_slots(int order, int half, int K)
+{
+    return half ? 2 * (K + 1) : (order >> 1) + K + (order & 1) + 1;
+}
+
+__global__ void k_loc_eval(const TfDesc *__restrict__ tfs, const LocBoxD *__restrict__ boxes,
+                           const LocItem *__restrict__ items, const double *__restrict__ coef,
+                           const double *__restrict__ cheb)
+{
+    extern __shared__ double sm[];
+    LocItem it = items[blockIdx.x];
+    LocBoxD bx = boxes[it.box];
+    const TfDesc &T = tfs[bx.tf];
+    const int K = T.K_loc;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    if (tid >= it.count) return;
+    double *S = sm + tid;   // slot k at S[k * nthr]
+    const int i = bx.i0 + it.start + tid;
+    const double w = T.w[i];
+    double scale = 1.0;
+    int boff;               // slot of b_0 (b_l at slot boff + l)
+    if (!T.half) {
+        const int l0 = T.order >> 1, odd = T.order & 1;
+        const int maxo = l0 + K + odd;
+        const double z = w * bx.r_end * 0.5;
+        if (z == 0.0) {
+            for (int k = 0; k <= maxo; ++k) S[k * nthr] = (k == 0) ? 1.0 : 0.0;
+        } else {
+            double m = (double)(maxo > (int)z ? maxo : (int)z);
+            int N = (int)(m + 20.0 + 4.0 * sqrt(m + 1.0) + 6.0 * cbrt(m + 1.0));
+            N += N & 1;
+            double jp1 = 0.0, j = 1e-280, sum = 0.0, tx = 2.0 / z;
+            for (int k = N; k >= 1; --k) {
+                double jm1 = (double)k * tx * j - jp1;
+                jp1 = j;
+                j = jm1;
+                int km1 = k - 1;
+                if (km1 <= maxo) S[km1 * nthr] = j;
+                if (km1 > 0 && !(km1 & 1)) sum += 2.0 * j;
+                if (fabs(j) > 1e200) {
+                    j *= 1e-200; jp1 *= 1e-200; sum *= 1e-200;
+                    for (int q = km1; q <= maxo; ++q) S[q * nthr] *= 1e-200;
+                }
+            }
+            double inv = 1.0 / (sum + j);
+            for (int k = 0; k <= maxo; ++k) S[k * nthr] *= inv;
+        }
+        // b_l replaces J_{l0+odd+l}; taken from the top down, every J still
+        // needed sits below the slots already replaced
+        for (int l = K; l >= 0; --l) {
+            int lo = l0 - l;
+            lo = lo < 0 ? -lo : lo;
+            double v = S[(l0 + odd + l) * nthr] * S[lo * nthr];
+            if (l > l0 && ((l - l0) & 1)) v = -v;
+            S[(l0 + odd + l) * nthr] = v;
+        }
+        boff = l0 + odd;
+    } else {
+        // f_q = j_n(z x_q), x_q = cos(pi (q + 1/2) / (2K + 2)), q = 0..K, at
+        // slots K+1+q; b_l at slots l
+        const double z = w * bx.r_end;
+        for (int q = 0; q <= K; ++q) {
+            double xq = cos(kPi * ((double)q + 0.5) / (double)(2 * K + 2));
+            double hi = z * xq, lo = fma(z, xq, -hi);
+            S[(K + 1 + q) * nthr] = sph_jn(T.order, hi, lo);
+        }
+        const double *M = cheb + T.cheb_off;
+        for (int l = 0; l <= K; ++l) {
+            double a = 0.0;
+            for (int q = 0; q <= K; ++q) a = fma(M[l * (K + 1) + q], S[(K + 1 + q) * nthr], a);
+            S[l * nthr] = a;
+        }
+        boff = 0;
+        scale = sqrt(2.0 * w / kPi);
+    }
+    const double sc = T.sign * scale;
+    const double *cf = coef + bx.coef_off;
+    const int B = T.B;
+    for (int t0 = 0; t0 < B; t0 += 4) {
+        const int nt = min(4, B - t0);
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (int l = 0; l <= K; ++l) {
+            double bl = S[(boff + l) * nthr];
+            const double *cr = cf + (long long)l * B + t0;
+            a0 = fma(bl, cr[0], a0);
+            if (nt > 1) a1 = fma(bl, cr[1], a1);
+            if (nt > 2) a2 = fma(bl, cr[2], a2);
+            if (nt > 3) a3 = fma(bl, cr[3], a3);
+        }
+        double *g = T.gs + (long long)t0 * T.ld_gs + i;
+        atomicAdd(g, sc * a0);
+        if (nt > 1) atomicAdd(g + T.ld_gs, sc * a1);
+        if (nt > 2) atomicAdd(g + 2 * (long long)T.ld_gs, sc * a2);
+        if (nt > 3) atomicAdd(g + 3 * (long long)T.ld_gs, sc * a3);
+    }
+}
 
 // Spread, source-dense boxes, on the matrix cores.  A block of 4 waves
 // covers kSP = 16 consecutive grid positions (wave w: positions 4w..4w+3 of
