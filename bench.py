@@ -218,6 +218,9 @@ def run_uneven(dev, tol: float, steps: int):
     xs = [torch.empty((b, t["r"].size), dtype=torch.float64, device=dev) for b, t in zip(Bs, ts)]
     colpts = 2.0 * sum(b * (t["r"].size + t["k"].size) for b, t in zip(Bs, ts))
     mf, ma = na.MultiBatch(pf, Bs), na.MultiBatch(pa, Bs)
+    t0 = time.time()
+    knobs = {"forward": mf.autotune(), "adjoint": ma.autotune()}
+    tune_s = time.time() - t0
 
     def batched():
         mf.execute(cs, gs, use_graph=True)
@@ -227,7 +230,8 @@ def run_uneven(dev, tol: float, steps: int):
         for p, c in zip(pf, cs):
             p.adjoint(p.forward(c.t()))
 
-    out = {"transforms": len(ts), "tol": tol, "plan_s": plan_s, "column_points": colpts}
+    out = {"transforms": len(ts), "tol": tol, "plan_s": plan_s, "autotune_s": tune_s,
+           "knobs": knobs, "column_points": colpts}
     for name, fn in (("batched", batched), ("plain", plain)):
         fn()
         torch.cuda.synchronize()

@@ -21,10 +21,11 @@ extern "C" {
 
 /* ---- precision policy ---------------------------------------------------
  * FP64: every stage in double precision (the reference's arithmetic).
- * AUTO (default): when tol >= 1e-6, spreading, interpolation, the uniform
- *   FFT and the local-expansion contraction run in FP32 with FP32
- *   accumulation; phases, deconvolution, Bessel evaluation and expansion
- *   coefficients stay FP64.  tol < 1e-6 runs FP64.
+ * AUTO (default): when tol >= 1e-6, the asymptotic stage's grids are FP32:
+ *   spreading multiplies and accumulates in FP32, the uniform FFT runs in
+ *   FP32 and interpolation reads FP32 grid values; phases, deconvolution,
+ *   the interpolation's products and sums, Bessel evaluation, the local
+ *   expansions and the direct sums stay FP64.  tol < 1e-6 runs FP64.
  * MIXED: the FP32 stages regardless of tol (for experiments). */
 enum { NUFHT_PREC_FP64 = 0, NUFHT_PREC_AUTO = 1, NUFHT_PREC_MIXED = 2 };
 int nufht_amd_plan_set_precision(NufhtPlan *plan, int mode);
@@ -108,10 +109,37 @@ typedef struct {
 } NufhtPlanStats;
 int nufht_amd_plan_stats(const NufhtPlan *plan, int batch, NufhtPlanStats *st);
 
-/* ---- autotuner --------------------------------------------------------------------
- * Times candidate tile shapes / grid oversampling on the device with the
- * plan's own geometry and keeps the fastest for this plan (and batch). */
+/* ---- launch-geometry knobs and the autotuner -----------------------------------
+ * Choices that change which kernels run and how work is cut into items, never
+ * the arithmetic's accuracy target:
+ *   dense_cell    sources per grid position from which an asymptotic box
+ *                 spreads on the matrix cores (else the VALU spread kernel)
+ *   dense_split   sources per matrix-core spread item
+ *   spread_split  sources per VALU spread item
+ *   interp_mfma   1: 16-target tiles whose kernel supports fit a 16-position
+ *                 window interpolate on the matrix cores; 0: VALU only
+ *   own_fft       1: two-pass FFT kernels for the lengths they cover; 0: rocFFT
+ *   precision     one of NUFHT_PREC_* (as nufht_amd_plan_set_precision)
+ * A plan starts from the defaults (NUFHT_AMD_DENSE_CELL and
+ * NUFHT_AMD_PRECISION override two of them). */
+typedef struct {
+    double dense_cell;
+    int dense_split, spread_split;
+    int interp_mfma, own_fft;
+    int precision;
+} NufhtAmdKnobs;
+int nufht_amd_get_knobs(const NufhtPlan *plan, NufhtAmdKnobs *k);
+int nufht_amd_set_knobs(NufhtPlan *plan, const NufhtAmdKnobs *k);
+
+/* Times the knobs on the device with the plan's own geometry and batch
+ * (coordinate descent: precision when tol >= 1e-6, then dense_cell,
+ * interp_mfma, dense_split, spread_split, own_fft; each candidate is built,
+ * run once untimed and timed over three calls) and keeps the fastest. */
 int nufht_amd_autotune(NufhtPlan *plan, int batch, void *stream);
+/* The same for a multi-transform batch (precision stays per plan); the
+ * launch sequence is rebuilt and, with graphs, captured again. */
+int nufht_multi_autotune(NufhtMulti *mt, void *stream);
+int nufht_multi_get_knobs(const NufhtMulti *mt, NufhtAmdKnobs *k);
 
 /* ---- diagnostics ---------------------------------------------------------------- */
 const char *nufht_amd_version(void);

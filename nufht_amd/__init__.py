@@ -85,6 +85,17 @@ class PlanStats(ctypes.Structure):
             "spread_sc_valu", "spread_sc_mfma", "interp_tc_valu", "interp_tc_mfma")]
 
 
+class Knobs(ctypes.Structure):
+    """NufhtAmdKnobs: launch-geometry choices (and the precision mode)."""
+    _fields_ = [("dense_cell", ctypes.c_double), ("dense_split", ctypes.c_int),
+                ("spread_split", ctypes.c_int), ("interp_mfma", ctypes.c_int),
+                ("own_fft", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+def _knobs_dict(k: Knobs) -> Dict[str, float]:
+    return {f: getattr(k, f) for f, _ in Knobs._fields_}
+
+
 # Every exported symbol of the C ABI, as declared in include/*.h.
 EXPORTS = [
     "nufht_default_options", "boxes_free", "nufht_plan_init", "nufht_plan_free",
@@ -95,7 +106,8 @@ EXPORTS = [
     "nufht_multi_create", "nufht_multi_execute", "nufht_multi_free",
     "nufht_amd_profile", "nufht_amd_profile_read", "nufht_amd_profile_reset",
     "nufht_amd_plan_stats", "nufht_amd_autotune", "nufht_amd_version",
-    "nufht_amd_last_error",
+    "nufht_amd_last_error", "nufht_amd_get_knobs", "nufht_amd_set_knobs",
+    "nufht_multi_autotune", "nufht_multi_get_knobs",
 ]
 
 
@@ -145,6 +157,10 @@ def lib():
                                          i, ctypes.POINTER(ctypes.c_char_p)]
     L.nufht_amd_plan_stats.argtypes = [PP, i, ctypes.POINTER(PlanStats)]
     L.nufht_amd_autotune.argtypes = [PP, i, vp]
+    L.nufht_amd_get_knobs.argtypes = [PP, ctypes.POINTER(Knobs)]
+    L.nufht_amd_set_knobs.argtypes = [PP, ctypes.POINTER(Knobs)]
+    L.nufht_multi_autotune.argtypes = [vp, vp]
+    L.nufht_multi_get_knobs.argtypes = [vp, ctypes.POINTER(Knobs)]
     L.nufht_amd_version.restype = ctypes.c_char_p
     L.nufht_amd_last_error.restype = ctypes.c_char_p
     _LIB = L
@@ -331,9 +347,31 @@ class Plan:
         _check(rc, "nufht_grad_nodes_device")
         return gr, gk
 
-    def autotune(self, batch: int, stream=None):
+    def autotune(self, batch: int, stream=None) -> Dict[str, float]:
+        """Times the launch-geometry knobs (and, at tol >= 1e-6, the precision
+        mode) for this plan and batch, keeps the fastest and returns them."""
         _check(lib().nufht_amd_autotune(ctypes.byref(self._plan), int(batch),
                                         _stream_handle(stream)), "nufht_amd_autotune")
+        return self.knobs()
+
+    def knobs(self) -> Dict[str, float]:
+        k = Knobs()
+        _check(lib().nufht_amd_get_knobs(ctypes.byref(self._plan), ctypes.byref(k)),
+               "nufht_amd_get_knobs")
+        return _knobs_dict(k)
+
+    def set_knobs(self, **kw):
+        """Overrides knobs by name (dense_cell, dense_split, spread_split,
+        interp_mfma, own_fft, precision); the next call rebuilds."""
+        k = Knobs()
+        _check(lib().nufht_amd_get_knobs(ctypes.byref(self._plan), ctypes.byref(k)),
+               "nufht_amd_get_knobs")
+        for name, v in kw.items():
+            if name not in dict(Knobs._fields_):
+                raise KeyError(name)
+            setattr(k, name, v)
+        _check(lib().nufht_amd_set_knobs(ctypes.byref(self._plan), ctypes.byref(k)),
+               "nufht_amd_set_knobs")
 
     def close(self):
         if self._live:
@@ -372,6 +410,20 @@ class MultiBatch:
         lg = (ctypes.c_int * K)(*[p.n for p in self.plans])
         _check(L.nufht_multi_execute(self._h, cp, lc, gp, lg, _stream_handle(stream),
                                      int(bool(use_graph))), "nufht_multi_execute")
+
+    def autotune(self, stream=None) -> Dict[str, float]:
+        """Times the launch-geometry knobs for the whole batch, keeps the
+        fastest (the program is rebuilt) and returns them."""
+        _check(lib().nufht_multi_autotune(self._h, _stream_handle(stream)),
+               "nufht_multi_autotune")
+        return self.knobs()
+
+    def knobs(self) -> Dict[str, float]:
+        k = Knobs()
+        _check(lib().nufht_multi_get_knobs(self._h, ctypes.byref(k)), "nufht_multi_get_knobs")
+        d = _knobs_dict(k)
+        d.pop("precision")
+        return d
 
     def close(self):
         if self._h:

@@ -251,11 +251,27 @@ static const int kSpreadSplit = 2048;
 static const int kDenseSplit = 4096;
 // sources per grid cell from which a box uses the split-K spread kernel
 static const double kDenseCell = 4.0;
-// (NUFHT_AMD_DENSE_CELL overrides the threshold, for tuning)
+// (NUFHT_AMD_DENSE_CELL overrides the threshold)
 static double g_dense_cell = [] {
     const char *e = getenv("NUFHT_AMD_DENSE_CELL");
     return e ? atof(e) : kDenseCell;
 }();
+
+// Launch-geometry choices of a program (NufhtAmdKnobs without precision).
+struct Knobs {
+    double dense_cell = g_dense_cell;
+    int dense_split = kDenseSplit;
+    int spread_split = kSpreadSplit;
+    int interp_mfma = 1;
+    int own_fft = 1;
+    bool operator==(const Knobs &o) const
+    {
+        return dense_cell == o.dense_cell && dense_split == o.dense_split &&
+               spread_split == o.spread_split && interp_mfma == o.interp_mfma &&
+               own_fft == o.own_fft;
+    }
+    bool operator!=(const Knobs &o) const { return !(*this == o); }
+};
 
 // FFT lengths restricted to 2^a and 3*2^a (prebuilt rocFFT kernels)
 static int fft_length(long long need)
@@ -359,6 +375,7 @@ struct FftClass {
 };
 
 struct Program {
+    Knobs knobs;      // set before build_program
     int prec = 0;     // 0 FP64, 2 mixed
     int emax = 0;
     std::vector<TfDesc> tfs;
@@ -401,9 +418,11 @@ struct EngineState {
     std::mutex mu;
     DevPlan dev;
     bool dev_ready = false;
+    Knobs knobs;            // the plan's choices
     Program *prog = nullptr;
     int prog_B = 0;
     int prog_prec = -1;
+    Knobs prog_knobs;       // the choices prog was built with
     // staging for host-pointer calls and permutations
     double *d_in = nullptr, *d_out = nullptr;
     size_t in_cap = 0, out_cap = 0;
@@ -674,7 +693,8 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         FftClass c;
         c.nf = kv.first;
         c.off = off;
-        c.own = fft_factor(c.nf, mixed ? fft_max_elems<float>() : fft_max_elems<double>(), c.p1,
+        c.own = P->knobs.own_fft &&
+                fft_factor(c.nf, mixed ? fft_max_elems<float>() : fft_max_elems<double>(), c.p1,
                            c.p2);
         own_nf[c.nf] = c.own;
         c.p1.chinfo_off = c.p2.chinfo_off = (int)P->fft_chinfo.size();
@@ -710,7 +730,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         int sp = 256 / std::max(1, tb / 2);            // spread positions per item
         const double *rb = h->r + A.j0, *re = h->r + A.j0 + A.nj;
         // source-dense boxes (many sources per grid cell) use the split-K kernel
-        bool dense = (double)A.nj >= g_dense_cell * (2.0 * A.L + 1.0);
+        bool dense = (double)A.nj >= P->knobs.dense_cell * (2.0 * A.L + 1.0);
         // sources whose kernel support reaches grid positions [s, s + np)
         auto src_range = [&](int s, int np) {
             int lmin = INT32_MAX, lmax = INT32_MIN;
@@ -754,13 +774,13 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                         sr.tile[t] = 4 * t < np ? src_range(s + 4 * t, std::min(4, np - 4 * t))
                                                 : make_int2(0, 0);
                     int jlo = sr.piece.x, jhi = sr.piece.y;
-                    if (jhi - jlo <= kDenseSplit) {
+                    if (jhi - jlo <= P->knobs.dense_split) {
                         P->adense.push_back({box, s, np, c0, tb, 0});
                         P->adrange.push_back(sr);
                     } else {
                         P->azero.push_back({box, s, np, c0, tb, 0});
-                        for (int j = jlo; j < jhi; j += kDenseSplit) {
-                            sr.piece = make_int2(j, std::min(jhi, j + kDenseSplit));
+                        for (int j = jlo; j < jhi; j += P->knobs.dense_split) {
+                            sr.piece = make_int2(j, std::min(jhi, j + P->knobs.dense_split));
                             P->adense.push_back({box, s, np, c0, tb, 1});
                             P->adrange.push_back(sr);
                         }
@@ -771,14 +791,14 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 for (int s = rg.first; s < rg.second; s += sp) {
                     int np = std::min(sp, rg.second - s);
                     int2 jr = src_range(s, np);
-                    if (jr.y - jr.x <= kSpreadSplit) {
+                    if (jr.y - jr.x <= P->knobs.spread_split) {
                         P->aspread.push_back({box, s, np, c0, tb, 0});
                         P->arange.push_back(jr);
                     } else {
                         P->azero.push_back({box, s, np, c0, tb, 0});
-                        for (int j = jr.x; j < jr.y; j += kSpreadSplit) {
+                        for (int j = jr.x; j < jr.y; j += P->knobs.spread_split) {
                             P->aspread.push_back({box, s, np, c0, tb, 1});
-                            P->arange.push_back(make_int2(j, std::min(jr.y, j + kSpreadSplit)));
+                            P->arange.push_back(make_int2(j, std::min(jr.y, j + P->knobs.spread_split)));
                         }
                     }
                 }
@@ -799,7 +819,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 double sf = h->w[A.i0 + s] - A.D, sl = h->w[A.i0 + s + cnt - 1] - A.D;
                 int m0f = (int)std::ceil((sf - A.a2) / A.delta);
                 int m0l = (int)std::ceil((sl - A.a2) / A.delta);
-                if (m0l - m0f + T.kw + 2 <= 16) {
+                if (P->knobs.interp_mfma && m0l - m0f + T.kw + 2 <= 16) {
                     flush(s);
                     P->aimf.push_back({box, s, cnt, c0, tb, m0f - 1});
                     P->stats.interp_tc_mfma += (long long)cnt * chans;
@@ -1094,16 +1114,18 @@ static int get_program(const NufhtPlan *plan, EngineState *E, int B, Program **o
 {
     RC_TRY(ensure_dev(plan, E));
     int prec = effective_prec(plan);
-    if (!E->prog || E->prog_B != B || E->prog_prec != prec) {
+    if (!E->prog || E->prog_B != B || E->prog_prec != prec || E->prog_knobs != E->knobs) {
         free_program(E->prog);
         E->prog = nullptr;
         Program *P = new Program();
+        P->knobs = E->knobs;
         std::vector<TfInput> in{{plan, B}};
         int rc = build_program(P, in);
         if (rc) { free_program(P); return rc; }
         E->prog = P;
         E->prog_B = B;
         E->prog_prec = prec;
+        E->prog_knobs = P->knobs;
     }
     *out = E->prog;
     return 0;
@@ -1405,6 +1427,7 @@ extern "C" int nufht_amd_plan_stats(const NufhtPlan *plan, int batch, NufhtPlanS
 struct NufhtMulti {
     std::vector<const NufhtPlan *> plans;
     std::vector<int> B;
+    Knobs knobs;
     Program *P = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -1427,9 +1450,28 @@ extern "C" int nufht_multi_create(NufhtMulti **out, int count, const NufhtPlan *
         in.push_back({plans[k], batch[k]});
     }
     mt->P = new Program();
+    mt->P->knobs = mt->knobs;
     int rc = build_program(mt->P, in);
     if (rc) { free_program(mt->P); delete mt; return rc; }
     *out = mt;
+    return 0;
+}
+
+// Rebuilds the multi-transform program with mt->knobs (dropping a captured
+// graph; the next graph launch captures again).
+static int multi_rebuild(NufhtMulti *mt)
+{
+    std::vector<TfInput> in;
+    for (size_t k = 0; k < mt->plans.size(); ++k) in.push_back({mt->plans[k], mt->B[k]});
+    Program *P = new Program();
+    P->knobs = mt->knobs;
+    int rc = build_program(P, in);
+    if (rc) { free_program(P); return rc; }
+    if (mt->gexec) { hipGraphExecDestroy(mt->gexec); mt->gexec = nullptr; }
+    if (mt->graph) { hipGraphDestroy(mt->graph); mt->graph = nullptr; }
+    mt->gkey.clear();
+    free_program(mt->P);
+    mt->P = P;
     return 0;
 }
 
@@ -1494,40 +1536,191 @@ extern "C" void nufht_multi_free(NufhtMulti *mt)
 }
 
 // ---------------------------------------------------------------------------
-// autotuner: the knob exposed today is the asymptotic-stage precision at
-// tol >= 1e-6 (FP32 vs FP64 grids); candidates are timed on the device with
-// the plan's own geometry and the faster one is kept.
+// knobs and the autotuner
+static void knobs_to_c(const Knobs &k, int prec, NufhtAmdKnobs *o)
+{
+    o->dense_cell = k.dense_cell;
+    o->dense_split = k.dense_split;
+    o->spread_split = k.spread_split;
+    o->interp_mfma = k.interp_mfma;
+    o->own_fft = k.own_fft;
+    o->precision = prec;
+}
+
+extern "C" int nufht_amd_get_knobs(const NufhtPlan *plan, NufhtAmdKnobs *k)
+{
+    if (!plan || !plan->impl || !k) return -400;
+    knobs_to_c(engine_of(plan)->knobs, ((NufhtImplHost *)plan->impl)->precision, k);
+    return 0;
+}
+
+static int knobs_check(const NufhtAmdKnobs *k)
+{
+    if (!(k->dense_cell > 0.0) || k->dense_split < kSQ || k->spread_split < 64 ||
+        (k->interp_mfma != 0 && k->interp_mfma != 1) || (k->own_fft != 0 && k->own_fft != 1) ||
+        (k->precision != NUFHT_PREC_FP64 && k->precision != NUFHT_PREC_AUTO &&
+         k->precision != NUFHT_PREC_MIXED)) {
+        nufht_set_error("invalid knobs");
+        return -400;
+    }
+    return 0;
+}
+
+extern "C" int nufht_amd_set_knobs(NufhtPlan *plan, const NufhtAmdKnobs *k)
+{
+    if (!plan || !plan->impl || !k) return -400;
+    RC_TRY(knobs_check(k));
+    EngineState *E = engine_of(plan);
+    E->knobs.dense_cell = k->dense_cell;
+    E->knobs.dense_split = k->dense_split;
+    E->knobs.spread_split = k->spread_split;
+    E->knobs.interp_mfma = k->interp_mfma;
+    E->knobs.own_fft = k->own_fft;
+    ((NufhtImplHost *)plan->impl)->precision = k->precision;
+    return 0;
+}
+
+extern "C" int nufht_multi_get_knobs(const NufhtMulti *mt, NufhtAmdKnobs *k)
+{
+    if (!mt || !k) return -400;
+    knobs_to_c(mt->knobs, NUFHT_PREC_AUTO, k);
+    return 0;
+}
+
+// Candidate values per knob, tried in this order by coordinate descent.
+static const double kTuneDense[] = {2.0, 4.0, 8.0, 1e300};
+static const int kTuneDenseSplit[] = {2048, 4096, 8192};
+static const int kTuneSpreadSplit[] = {1024, 2048, 4096};
+
+// Times `run` (which enqueues one call on st): one untimed call (builds the
+// program), then the mean of three.  Returns a negative code on failure.
+template <class Run> static int tune_time(Run run, hipStream_t st, float *ms)
+{
+    RC_TRY(run());
+    HIP_TRY(hipStreamSynchronize(st));
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, st));
+    int rc = 0;
+    for (int r = 0; r < 3 && !rc; ++r) rc = run();
+    HIP_TRY(hipEventRecord(b, st));
+    HIP_TRY(hipEventSynchronize(b));
+    float t = 0.f;
+    hipEventElapsedTime(&t, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms = t / 3.0f;
+    return rc;
+}
+
+// Coordinate descent over the knobs of `cur` with `apply` (sets knobs, which
+// the next call builds) and `run`; a candidate replaces the incumbent when it
+// is at least 2 % faster.
+template <class Apply, class Run>
+static int tune_knobs(Knobs &cur, Apply apply, Run run, hipStream_t st)
+{
+    float best;
+    apply(cur);
+    RC_TRY(tune_time(run, st, &best));
+    auto consider = [&](Knobs cand) -> int {
+        if (cand == cur) return 0;
+        apply(cand);
+        float ms;
+        int rc = tune_time(run, st, &ms);
+        if (rc == 0 && ms < 0.98f * best) {
+            best = ms;
+            cur = cand;
+        }
+        apply(cur);
+        return 0;
+    };
+    for (double v : kTuneDense) { Knobs c = cur; c.dense_cell = v; consider(c); }
+    for (int v = 1; v >= 0; --v) { Knobs c = cur; c.interp_mfma = v; consider(c); }
+    for (int v : kTuneDenseSplit) { Knobs c = cur; c.dense_split = v; consider(c); }
+    for (int v : kTuneSpreadSplit) { Knobs c = cur; c.spread_split = v; consider(c); }
+    for (int v = 1; v >= 0; --v) { Knobs c = cur; c.own_fft = v; consider(c); }
+    apply(cur);
+    return 0;
+}
+
 extern "C" int nufht_amd_autotune(NufhtPlan *plan, int batch, void *stream)
 {
     if (!plan || !plan->impl || batch <= 0) return -400;
     NufhtImplHost *h = (NufhtImplHost *)plan->impl;
-    if (plan->tol < 1e-6) return 0;
+    EngineState *E = engine_of(plan);
     hipStream_t st = (hipStream_t)stream;
     double *dc = nullptr, *dg = nullptr;
     HIP_TRY(hipMalloc((void **)&dc, sizeof(double) * plan->m * batch));
     HIP_TRY(hipMalloc((void **)&dg, sizeof(double) * plan->n * batch));
     HIP_TRY(hipMemset(dc, 0, sizeof(double) * plan->m * batch));
-    int modes[2] = {NUFHT_PREC_FP64, NUFHT_PREC_MIXED};
-    float best = 1e30f;
-    int best_mode = h->precision;
-    for (int q = 0; q < 2; ++q) {
-        h->precision = modes[q];
-        hipEvent_t a, b;
-        hipEventCreate(&a);
-        hipEventCreate(&b);
-        int rc = nufht_batch_device(plan, dc, plan->m, dg, plan->n, batch, st);  // warm-up / build
-        hipEventRecord(a, st);
-        for (int r = 0; r < 3 && !rc; ++r) rc = nufht_batch_device(plan, dc, plan->m, dg, plan->n, batch, st);
-        hipEventRecord(b, st);
-        hipEventSynchronize(b);
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, a, b);
-        hipEventDestroy(a);
-        hipEventDestroy(b);
-        if (!rc && ms < best) { best = ms; best_mode = modes[q]; }
+    auto run = [&]() { return nufht_batch_device(plan, dc, plan->m, dg, plan->n, batch, st); };
+    int rc = 0;
+    // precision first (both meet tol at tol >= 1e-6), then the geometry
+    if (plan->tol >= 1e-6) {
+        int modes[2] = {NUFHT_PREC_MIXED, NUFHT_PREC_FP64};
+        float best = 1e30f;
+        int best_mode = h->precision;
+        for (int q = 0; q < 2 && !rc; ++q) {
+            h->precision = modes[q];
+            float ms;
+            rc = tune_time(run, st, &ms);
+            if (!rc && ms < best) { best = ms; best_mode = modes[q]; }
+        }
+        h->precision = best_mode;
     }
-    h->precision = best_mode;
+    if (!rc) {
+        Knobs cur = E->knobs;
+        rc = tune_knobs(cur, [&](const Knobs &k) { E->knobs = k; }, run, st);
+    }
     hipFree(dc);
     hipFree(dg);
-    return 0;
+    return rc;
+}
+
+extern "C" int nufht_multi_autotune(NufhtMulti *mt, void *stream)
+{
+    if (!mt) return -400;
+    hipStream_t st = (hipStream_t)stream;
+    size_t n = mt->plans.size();
+    std::vector<double *> dc(n, nullptr), dg(n, nullptr);
+    std::vector<const double *> pc(n);
+    std::vector<double *> pg(n);
+    std::vector<int> lc(n), lg(n);
+    int rc = 0;
+    for (size_t k = 0; k < n && !rc; ++k) {
+        const NufhtPlan *p = mt->plans[k];
+        if (hipMalloc((void **)&dc[k], sizeof(double) * p->m * mt->B[k]) != hipSuccess ||
+            hipMalloc((void **)&dg[k], sizeof(double) * p->n * mt->B[k]) != hipSuccess ||
+            hipMemset(dc[k], 0, sizeof(double) * p->m * mt->B[k]) != hipSuccess) {
+            nufht_set_error("autotune buffers: out of device memory");
+            rc = -510;
+        }
+        pc[k] = dc[k];
+        pg[k] = dg[k];
+        lc[k] = p->m;
+        lg[k] = p->n;
+    }
+    if (!rc) {
+        auto run = [&]() {
+            return nufht_multi_execute(mt, pc.data(), lc.data(), pg.data(), lg.data(), st, 0);
+        };
+        Knobs cur = mt->knobs;
+        int brc = 0;
+        rc = tune_knobs(cur,
+                        [&](const Knobs &k) {
+                            if (k != mt->knobs || !mt->P) {
+                                mt->knobs = k;
+                                int r = multi_rebuild(mt);
+                                if (r && !brc) brc = r;
+                            }
+                        },
+                        run, st);
+        if (!rc) rc = brc;
+    }
+    for (size_t k = 0; k < n; ++k) {
+        hipFree(dc[k]);
+        hipFree(dg[k]);
+    }
+    return rc;
 }
