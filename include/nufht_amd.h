@@ -141,6 +141,17 @@ int nufht_amd_autotune(NufhtPlan *plan, int batch, void *stream);
 int nufht_multi_autotune(NufhtMulti *mt, void *stream);
 int nufht_multi_get_knobs(const NufhtMulti *mt, NufhtAmdKnobs *k);
 
+/* ---- plan cache -------------------------------------------------------------------
+ * No counterpart in the reference, which plans on every call.  The device
+ * state of a freed plan (sorted nodes, launch program, grids, FFT plans,
+ * tuned knobs) is kept for reuse by the next plan with the same order,
+ * options, sizes and node arrays (a 128-bit hash of r and of k), so that
+ * callers that plan every time (the _pynufht entry points) pay the device
+ * set-up once.  Keeps the most recent `capacity` states when capacity >= 0
+ * (default 2, NUFHT_AMD_PLAN_CACHE; 0 frees them and disables the cache);
+ * *hits (may be NULL) receives the number of reuses so far. */
+int nufht_amd_plan_cache(int capacity, long long *hits);
+
 /* ---- diagnostics ---------------------------------------------------------------- */
 const char *nufht_amd_version(void);
 const char *nufht_amd_last_error(void);

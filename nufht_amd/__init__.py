@@ -107,7 +107,7 @@ EXPORTS = [
     "nufht_amd_profile", "nufht_amd_profile_read", "nufht_amd_profile_reset",
     "nufht_amd_plan_stats", "nufht_amd_autotune", "nufht_amd_version",
     "nufht_amd_last_error", "nufht_amd_get_knobs", "nufht_amd_set_knobs",
-    "nufht_multi_autotune", "nufht_multi_get_knobs",
+    "nufht_multi_autotune", "nufht_multi_get_knobs", "nufht_amd_plan_cache",
 ]
 
 
@@ -161,10 +161,19 @@ def lib():
     L.nufht_amd_set_knobs.argtypes = [PP, ctypes.POINTER(Knobs)]
     L.nufht_multi_autotune.argtypes = [vp, vp]
     L.nufht_multi_get_knobs.argtypes = [vp, ctypes.POINTER(Knobs)]
+    L.nufht_amd_plan_cache.argtypes = [i, ctypes.POINTER(ctypes.c_longlong)]
     L.nufht_amd_version.restype = ctypes.c_char_p
     L.nufht_amd_last_error.restype = ctypes.c_char_p
     _LIB = L
     return L
+
+
+def plan_cache(capacity: int = -1) -> int:
+    """Device-state cache of freed plans: sets its capacity when capacity >= 0
+    (0 empties and disables it) and returns the number of reuses so far."""
+    hits = ctypes.c_longlong()
+    _check(lib().nufht_amd_plan_cache(int(capacity), ctypes.byref(hits)), "nufht_amd_plan_cache")
+    return hits.value
 
 
 def last_error() -> str:

@@ -414,8 +414,17 @@ struct Program {
     bool desc_dirty = true;
 };
 
+// Identity of a plan's device state: order, options, sizes, device and
+// 128 bits of hash of the caller's r and k arrays.
+struct StateKey {
+    double nu, tol, z_split;
+    int max_levels, min_dim_prod, K_asy, K_loc, m, n, device;
+    unsigned long long hr[2], hw[2];
+};
+
 struct EngineState {
     std::mutex mu;
+    StateKey key;
     DevPlan dev;
     bool dev_ready = false;
     Knobs knobs;            // the plan's choices
@@ -471,10 +480,80 @@ static void free_program(Program *P)
     delete P;
 }
 
+// Two independent 64-bit mixes (splitmix64 finaliser per word, chained).
+static void hash_doubles(const double *x, int n, unsigned long long out[2])
+{
+    unsigned long long a = 0x243F6A8885A308D3ull ^ (unsigned long long)n,
+                       b = 0x13198A2E03707344ull + (unsigned long long)n;
+    for (int i = 0; i < n; ++i) {
+        unsigned long long v;
+        memcpy(&v, x + i, sizeof v);
+        unsigned long long z = v + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        a = (a ^ z) * 0x100000001B3ull;
+        a ^= a >> 29;
+        b = (b + (z ^ (v >> 17))) * 0xD6E8FEB86659FD93ull;
+        b ^= b >> 32;
+    }
+    out[0] = a;
+    out[1] = b;
+}
+
+static StateKey state_key(const NufhtPlan *plan)
+{
+    const NufhtImplHost *h = (const NufhtImplHost *)plan->impl;
+    StateKey k;
+    memset(&k, 0, sizeof k);
+    k.nu = plan->nu;
+    k.tol = h->opt.tol;
+    k.z_split = h->opt.z_split;
+    k.max_levels = h->opt.max_levels;
+    k.min_dim_prod = h->opt.min_dim_prod;
+    k.K_asy = h->opt.K_asy;
+    k.K_loc = h->opt.K_loc;
+    k.m = plan->m;
+    k.n = plan->n;
+    hipGetDevice(&k.device);
+    hash_doubles(plan->rs, plan->m, k.hr);
+    hash_doubles(plan->ws, plan->n, k.hw);
+    return k;
+}
+
+// Released device states kept for reuse by a later plan with the same key
+// (the reference's Python entry points plan every call); most recent first,
+// at most NUFHT_AMD_PLAN_CACHE entries (default 2, 0 disables).
+static std::mutex g_cache_mu;
+static std::vector<EngineState *> *g_cache = new std::vector<EngineState *>();
+static int g_cache_cap = [] {
+    const char *e = getenv("NUFHT_AMD_PLAN_CACHE");
+    return e ? std::max(0, atoi(e)) : 2;
+}();
+static long long g_cache_hits = 0;
+
 static EngineState *engine_of(const NufhtPlan *plan)
 {
     NufhtImplHost *h = (NufhtImplHost *)plan->impl;
-    if (!h->dev) h->dev = new EngineState();
+    if (!h->dev) {
+        StateKey k = state_key(plan);
+        EngineState *E = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(g_cache_mu);
+            for (size_t q = 0; q < g_cache->size(); ++q)
+                if (!memcmp(&(*g_cache)[q]->key, &k, sizeof k)) {
+                    E = (*g_cache)[q];
+                    g_cache->erase(g_cache->begin() + q);
+                    g_cache_hits++;
+                    break;
+                }
+        }
+        if (!E) {
+            E = new EngineState();
+            E->key = k;
+        }
+        h->dev = E;
+    }
     return (EngineState *)h->dev;
 }
 
@@ -498,16 +577,50 @@ static int ensure_dev(const NufhtPlan *plan, EngineState *E)
     return 0;
 }
 
-extern "C" void nufht_engine_release(NufhtImplHost *h)
+static void engine_destroy(EngineState *E)
 {
-    EngineState *E = (EngineState *)h->dev;
-    if (!E) return;
     free_program(E->prog);
     hipFree(E->dev.d_r); hipFree(E->dev.d_w); hipFree(E->dev.d_perm_r); hipFree(E->dev.d_perm_w);
     hipFree(E->d_in); hipFree(E->d_out);
     for (int i = 0; i < 6; ++i) hipFree(E->d_tmp[i]);
     delete E;
+}
+
+extern "C" void nufht_engine_release(NufhtImplHost *h)
+{
+    EngineState *E = (EngineState *)h->dev;
+    if (!E) return;
     h->dev = nullptr;
+    std::vector<EngineState *> drop;
+    if (E->prog && g_cache_cap > 0) {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_cache->insert(g_cache->begin(), E);
+        while ((int)g_cache->size() > g_cache_cap) {
+            drop.push_back(g_cache->back());
+            g_cache->pop_back();
+        }
+    } else {
+        drop.push_back(E);
+    }
+    for (EngineState *d : drop) engine_destroy(d);
+}
+
+extern "C" int nufht_amd_plan_cache(int capacity, long long *hits)
+{
+    std::vector<EngineState *> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        if (hits) *hits = g_cache_hits;
+        if (capacity >= 0) {
+            g_cache_cap = capacity;
+            while ((int)g_cache->size() > g_cache_cap) {
+                drop.push_back(g_cache->back());
+                g_cache->pop_back();
+            }
+        }
+    }
+    for (EngineState *d : drop) engine_destroy(d);
+    return 0;
 }
 
 extern "C" void nufht_engine_scratch_release(NufhtScratch *s) { (void)s; }

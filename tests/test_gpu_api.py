@@ -183,3 +183,35 @@ def test_uneven_multi_batch_and_graph_replays(device):
     mb.close()
     for p in plans:
         p.close()
+
+
+def test_plan_cache_reuses_device_state(device):
+    import time
+    import nufht_amd as na
+    pn = na.load_pynufht()
+    r, k, cs = _uniform(3, 20000, 20000, 200.0, 200.0, 2)
+    rows = np.arange(0, 20000, 397)
+    ref = oracle.direct_rows(2.0, r, cs, k, rows)
+    h0 = na.plan_cache()
+    say("_pynufht.nufht_batch twice on the same nodes")
+    t0 = time.perf_counter()
+    g1 = pn.nufht_batch(2.0, r, cs, k, tol=1e-8)
+    t1 = time.perf_counter()
+    g2 = pn.nufht_batch(2.0, r, cs, k, tol=1e-8)
+    t2 = time.perf_counter()
+    say(f"  first call {t1 - t0:.3f} s, second {t2 - t1:.3f} s")
+    assert na.plan_cache() == h0 + 1
+    assert rel_l2(g1[rows], ref) < 2e-8
+    assert rel_l2(g2, g1) < 1e-13
+    say("other nodes or order: no reuse")
+    r2 = r.copy()
+    r2[5] = 0.5 * (r2[4] + r2[5])
+    g3 = pn.nufht_batch(2.0, r2, cs, k, tol=1e-8)
+    g4 = pn.nufht_batch(3.0, r, cs, k, tol=1e-8)
+    assert na.plan_cache() == h0 + 1
+    assert rel_l2(g3[rows], oracle.direct_rows(2.0, r2, cs, k, rows)) < 2e-8
+    assert rel_l2(g4[rows], oracle.direct_rows(3.0, r, cs, k, rows)) < 2e-8
+    na.plan_cache(0)
+    g5 = pn.nufht_batch(2.0, r, cs, k, tol=1e-8)
+    assert na.plan_cache(2) == h0 + 1
+    assert rel_l2(g5, g1) < 1e-13
