@@ -141,6 +141,14 @@ int nufht_amd_autotune(NufhtPlan *plan, int batch, void *stream);
 int nufht_multi_autotune(NufhtMulti *mt, void *stream);
 int nufht_multi_get_knobs(const NufhtMulti *mt, NufhtAmdKnobs *k);
 
+/* ---- table cache writer ------------------------------------------------------------
+ * Replaces the reference's table generation and cache save
+ * (src/bounds.c:306-324, src/fast_hankel_nufht.c:193-247): computes every
+ * entry the reference's generator fills and writes them in its cache format
+ * (the file NUFHT_TABLE_CACHE may name, and the reference's
+ * data/nufht_tables.cache).  Returns 0, -1 (memory) or -2 (I/O). */
+int nufht_amd_write_table_cache(const char *path);
+
 /* ---- plan cache -------------------------------------------------------------------
  * No counterpart in the reference, which plans on every call.  The device
  * state of a freed plan (sorted nodes, launch program, grids, FFT plans,

@@ -108,6 +108,7 @@ EXPORTS = [
     "nufht_amd_plan_stats", "nufht_amd_autotune", "nufht_amd_version",
     "nufht_amd_last_error", "nufht_amd_get_knobs", "nufht_amd_set_knobs",
     "nufht_multi_autotune", "nufht_multi_get_knobs", "nufht_amd_plan_cache",
+    "nufht_amd_write_table_cache",
 ]
 
 
@@ -162,6 +163,7 @@ def lib():
     L.nufht_multi_autotune.argtypes = [vp, vp]
     L.nufht_multi_get_knobs.argtypes = [vp, ctypes.POINTER(Knobs)]
     L.nufht_amd_plan_cache.argtypes = [i, ctypes.POINTER(ctypes.c_longlong)]
+    L.nufht_amd_write_table_cache.argtypes = [ctypes.c_char_p]
     L.nufht_amd_version.restype = ctypes.c_char_p
     L.nufht_amd_last_error.restype = ctypes.c_char_p
     _LIB = L

@@ -99,7 +99,9 @@ static int wimp_terms(double nu, double z, double tol)
 {
     int k = (int)ceil((z - nu) / 2.0);
     if (k < 1) k = 1;
-    while (!(wimp_bound(nu, k, z) < tol) && k < 100000) k++;
+    /* no cap: the K = 1 slots of small tolerances need ~1e5 terms (never
+     * used by a plan, whose K_asy is at least 2, but written to the cache) */
+    while (!(wimp_bound(nu, k, z) < tol) && k < (1 << 30)) k++;
     return k;
 }
 
@@ -162,4 +164,54 @@ int nufht_tables_row(double abs_nu, double *coef64)
 {
     for (int J = 0; J < 64; ++J) coef64[J] = (J <= 21) ? a_coef(J, abs_nu) : 0.0;
     return 0;
+}
+
+/* Writes every table entry the reference's generator fills (a_k for
+ * nu = 0, 1/2, ..., 200 and k <= 21; z and K_loc for integer nu = 0..200,
+ * the 12 tolerances and K = 1..10 at slots 0..9) in the reference's cache
+ * format (src/fast_hankel_nufht.c:186-247): a header of six uint32
+ * {0x4E5546, 1, 401, 401, 32, 16}, then double asy_coef[401][64], double
+ * z[401][32][16], int32 K[401][32][16], native endian, every other entry
+ * zero (slot 10 too, as the reference leaves it).  Returns 0, -1 on an
+ * allocation failure, -2 if the file cannot be written. */
+int nufht_amd_write_table_cache(const char *path)
+{
+    const size_t nz = 401 * 32 * 16;
+    double *coef = (double *)calloc(401 * 64, sizeof(double));
+    double *z = (double *)calloc(nz, sizeof(double));
+    int32_t *k = (int32_t *)calloc(nz, sizeof(int32_t));
+    int rc = 0;
+    if (!coef || !z || !k) {
+        rc = -1;
+        goto out;
+    }
+    for (int i = 0; i < 401; ++i) nufht_tables_row(0.5 * i, coef + (size_t)i * 64);
+    for (int nu = 0; nu < T_NU; ++nu)
+        for (int j = 0; j < T_TOL; ++j)
+            for (int slot = 0; slot < 10; ++slot) {
+                double zz;
+                int kk;
+                nufht_tables_lookup(nu, j, slot, &zz, &kk);
+                size_t at = ((size_t)nu * 32 + j) * 16 + slot;
+                z[at] = zz;
+                k[at] = kk;
+            }
+    {
+        const uint32_t hdr[6] = {0x4E5546u, 1u, 401u, 401u, 32u, 16u};
+        FILE *f = path ? fopen(path, "wb") : NULL;
+        if (!f) {
+            rc = -2;
+            goto out;
+        }
+        if (fwrite(hdr, sizeof(uint32_t), 6, f) != 6 ||
+            fwrite(coef, sizeof(double), 401 * 64, f) != 401 * 64 ||
+            fwrite(z, sizeof(double), nz, f) != nz || fwrite(k, sizeof(int32_t), nz, f) != nz)
+            rc = -2;
+        if (fclose(f) != 0) rc = -2;
+    }
+out:
+    free(coef);
+    free(z);
+    free(k);
+    return rc;
 }
