@@ -119,6 +119,10 @@ int nufht_amd_plan_stats(const NufhtPlan *plan, int batch, NufhtPlanStats *st);
  *   interp_mfma   1: 16-target tiles whose kernel supports fit a 16-position
  *                 window interpolate on the matrix cores; 0: VALU only
  *   own_fft       1: two-pass FFT kernels for the lengths they cover; 0: rocFFT
+ *   spread_sched  matrix-core spread schedule: 1 split (4 waves stage source
+ *                 chunks into double-buffered LDS while 4 multiply), 0 unified
+ *                 (4 waves stage, then multiply), -1 automatic (split for FP64
+ *                 grids with at least 3 exponent groups)
  *   precision     one of NUFHT_PREC_* (as nufht_amd_plan_set_precision)
  * A plan starts from the defaults (NUFHT_AMD_DENSE_CELL and
  * NUFHT_AMD_PRECISION override two of them). */
@@ -127,13 +131,14 @@ typedef struct {
     int dense_split, spread_split;
     int interp_mfma, own_fft;
     int precision;
+    int spread_sched;
 } NufhtAmdKnobs;
 int nufht_amd_get_knobs(const NufhtPlan *plan, NufhtAmdKnobs *k);
 int nufht_amd_set_knobs(NufhtPlan *plan, const NufhtAmdKnobs *k);
 
 /* Times the knobs on the device with the plan's own geometry and batch
  * (coordinate descent: precision when tol >= 1e-6, then dense_cell,
- * interp_mfma, dense_split, spread_split, own_fft; each candidate is built,
+ * interp_mfma, dense_split, spread_split, own_fft, spread_sched; each candidate is built,
  * run once untimed and timed over three calls) and keeps the fastest. */
 int nufht_amd_autotune(NufhtPlan *plan, int batch, void *stream);
 /* The same for a multi-transform batch (precision stays per plan); the

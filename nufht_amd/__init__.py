@@ -89,7 +89,8 @@ class Knobs(ctypes.Structure):
     """NufhtAmdKnobs: launch-geometry choices (and the precision mode)."""
     _fields_ = [("dense_cell", ctypes.c_double), ("dense_split", ctypes.c_int),
                 ("spread_split", ctypes.c_int), ("interp_mfma", ctypes.c_int),
-                ("own_fft", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("own_fft", ctypes.c_int), ("precision", ctypes.c_int),
+                ("spread_sched", ctypes.c_int)]
 
 
 def _knobs_dict(k: Knobs) -> Dict[str, float]:

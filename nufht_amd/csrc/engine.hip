@@ -264,11 +264,12 @@ struct Knobs {
     int spread_split = kSpreadSplit;
     int interp_mfma = 1;
     int own_fft = 1;
+    int spread_sched = -1;   // matrix-core spread schedule: -1 automatic, 0 unified, 1 split
     bool operator==(const Knobs &o) const
     {
         return dense_cell == o.dense_cell && dense_split == o.dense_split &&
                spread_split == o.spread_split && interp_mfma == o.interp_mfma &&
-               own_fft == o.own_fft;
+               own_fft == o.own_fft && spread_sched == o.spread_sched;
     }
     bool operator!=(const Knobs &o) const { return !(*this == o); }
 };
@@ -376,6 +377,7 @@ struct FftClass {
 
 struct Program {
     Knobs knobs;      // set before build_program
+    bool spread_split = false;   // matrix-core spread schedule (resolved from knobs)
     int prec = 0;     // 0 FP64, 2 mixed
     int emax = 0;
     std::vector<TfDesc> tfs;
@@ -833,6 +835,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         P->fft.push_back(c);
     }
     P->grid_elems = off;
+    int g_dense_max = 0;   // exponent groups of the matrix-core spread's boxes
     for (auto &ap : apend) {
         int box = (int)P->abox.size();
         const AsyBoxD &A = ap.b;
@@ -844,6 +847,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         const double *rb = h->r + A.j0, *re = h->r + A.j0 + A.nj;
         // source-dense boxes (many sources per grid cell) use the split-K kernel
         bool dense = (double)A.nj >= P->knobs.dense_cell * (2.0 * A.L + 1.0);
+        if (dense) {
+            int nsub = 16 / tb;
+            g_dense_max = std::max(g_dense_max, (A.E + 4 * nsub - 1) / (4 * nsub));
+        }
         // sources whose kernel support reaches grid positions [s, s + np)
         auto src_range = [&](int s, int np) {
             int lmin = INT32_MAX, lmax = INT32_MIN;
@@ -947,6 +954,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     }
     P->stats.fft_classes = (long long)P->fft.size();
     P->emax = e_max <= 6 ? 6 : e_max <= 10 ? 10 : e_max <= 14 ? 14 : 22;
+    // split schedule of the matrix-core spread where its chunks carry enough
+    // products: FP64 grids with at least 3 exponent groups
+    P->spread_split = P->knobs.spread_sched >= 0 ? P->knobs.spread_sched == 1
+                                                 : (!mixed && g_dense_max >= 3);
     // local-expansion launch geometry: projection items of loc_ch sources x
     // 16 columns with ceil((K+1)/16) row tiles; evaluation items of loc_tpb
     // targets x all columns, the per-target scratch within 48 KiB of LDS
@@ -1101,9 +1112,25 @@ static void launch_asy(Program *P, hipStream_t st)
     }
     if (!P->adense.empty()) {
         StageTimer tm(ST_SPREAD_MF, st);
-        hipLaunchKernelGGL((k_asy_spread_mfma<R, EM>), dim3((unsigned)P->adense.size()),
-                           dim3(256), 0, st, P->d_tfs, P->d_abox, P->d_adense, P->d_adrange,
-                           (G *)P->d_grid, P->d_ft);
+        static std::once_flag attr_once;
+        std::call_once(attr_once, []() {
+            hipFuncSetAttribute((const void *)k_asy_spread_mfma<R, EM, true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)SpreadSched<true>::template lds<R, EM>());
+            hipFuncSetAttribute((const void *)k_asy_spread_mfma<R, EM, false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)SpreadSched<false>::template lds<R, EM>());
+        });
+        const size_t lds_split = SpreadSched<true>::template lds<R, EM>();
+        const size_t lds_unified = SpreadSched<false>::template lds<R, EM>();
+        if (P->spread_split)
+            hipLaunchKernelGGL((k_asy_spread_mfma<R, EM, true>), dim3((unsigned)P->adense.size()),
+                               dim3(SpreadSched<true>::threads), lds_split, st, P->d_tfs,
+                               P->d_abox, P->d_adense, P->d_adrange, (G *)P->d_grid, P->d_ft);
+        else
+            hipLaunchKernelGGL((k_asy_spread_mfma<R, EM, false>), dim3((unsigned)P->adense.size()),
+                               dim3(SpreadSched<false>::threads), lds_unified, st, P->d_tfs,
+                               P->d_abox, P->d_adense, P->d_adrange, (G *)P->d_grid, P->d_ft);
     }
     {
         StageTimer tm(ST_FFT, st);
@@ -1657,6 +1684,7 @@ static void knobs_to_c(const Knobs &k, int prec, NufhtAmdKnobs *o)
     o->spread_split = k.spread_split;
     o->interp_mfma = k.interp_mfma;
     o->own_fft = k.own_fft;
+    o->spread_sched = k.spread_sched;
     o->precision = prec;
 }
 
@@ -1671,6 +1699,7 @@ static int knobs_check(const NufhtAmdKnobs *k)
 {
     if (!(k->dense_cell > 0.0) || k->dense_split < kSQ || k->spread_split < 64 ||
         (k->interp_mfma != 0 && k->interp_mfma != 1) || (k->own_fft != 0 && k->own_fft != 1) ||
+        k->spread_sched < -1 || k->spread_sched > 1 ||
         (k->precision != NUFHT_PREC_FP64 && k->precision != NUFHT_PREC_AUTO &&
          k->precision != NUFHT_PREC_MIXED)) {
         nufht_set_error("invalid knobs");
@@ -1689,6 +1718,7 @@ extern "C" int nufht_amd_set_knobs(NufhtPlan *plan, const NufhtAmdKnobs *k)
     E->knobs.spread_split = k->spread_split;
     E->knobs.interp_mfma = k->interp_mfma;
     E->knobs.own_fft = k->own_fft;
+    E->knobs.spread_sched = k->spread_sched;
     ((NufhtImplHost *)plan->impl)->precision = k->precision;
     return 0;
 }
@@ -1753,6 +1783,7 @@ static int tune_knobs(Knobs &cur, Apply apply, Run run, hipStream_t st)
     for (int v : kTuneDenseSplit) { Knobs c = cur; c.dense_split = v; consider(c); }
     for (int v : kTuneSpreadSplit) { Knobs c = cur; c.spread_split = v; consider(c); }
     for (int v = 1; v >= 0; --v) { Knobs c = cur; c.own_fft = v; consider(c); }
+    for (int v = -1; v <= 1; ++v) { Knobs c = cur; c.spread_sched = v; consider(c); }
     apply(cur);
     return 0;
 }

@@ -789,77 +789,61 @@ __device__ __forceinline__ void spread_chunk(const SpreadLds<R, EMAX> &L, int ql
 // psi u^((4g+e1) nsub), B carries c (rho/r)^{e2}.  Re and Im are separate
 // products sharing B.  Per source, the phase, powers, kernel values and the
 // B operand are computed once per block and shared through LDS.
-template <typename R, int EMAX, int NG>
-__device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD &bx,
-                                                 const AsyItem &it, int2 piece, int2 trg,
-                                                 typename Cplx<R>::T *__restrict__ grid,
-                                                 const double *__restrict__ ft,
-                                                 SpreadLds<R, EMAX> &L)
+// Staging of one chunk of up to kSQ sources by the 4 producer waves
+// (pw = producer wave, ptid = its thread index in 0..255): wave 0 the
+// phases and powers, wave 1 the kernel values at the block's positions,
+// waves 2-3 the B operand.  Rows past the chunk's end are zero.
+template <typename R, int EMAX>
+__device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx, const AsyItem &it,
+                                             int c0, int cn, int pw, int ptid,
+                                             const double *__restrict__ ft, SpreadLds<R, EMAX> &L)
 {
     typedef typename Cplx<R>::T C2;
-    typedef typename Mfma<R>::V V;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int tb = it.tb, nsub = 16 / tb;
-    const int E = bx.E, kw = T.kw;
-    const double inv_h = 1.0 / bx.h;
-    const int ka = lane >> 4, ia = lane & 15;
-    const int pa = ia >> 2, e1a = ia & 3;
-    const int jb = lane & 15;
-    // the block's first position as an l value; positions past the item or
-    // outside [-L, L] get no kernel values
-    const int qa = max(trg.x, piece.x), qb = min(trg.y, piece.y);
-    V acc_r[NG], acc_i[NG];
+    const int lane = ptid & 63, tb = it.tb, E = bx.E, kw = T.kw;
+    if (pw == 0) {
+        const int q = lane;
+        double rj = q < cn ? T.r[c0 + q] : bx.C;
+        double x = rj - bx.C;
+        double sn, cs;
+        sincos_cw(bx.D * x, &sn, &cs);
+        double rs = q < cn ? 1.0 / sqrt(rj) : 0.0;
+        double br = cs * rs, bi = sn * rs;
+        double ri = q < cn ? bx.rho / rj : 0.0, rp = 1.0;
+        for (int e = 0; e < E; ++e) {
+            C2 u;
+            u.x = (R)(br * rp);
+            u.y = (R)(bi * rp);
+            L.u[q][e] = u;
+            rp *= ri;
+        }
+    } else if (pw == 1) {
+        const int q = lane;
+        const double inv_h = 1.0 / bx.h;
 #pragma unroll
-    for (int g = 0; g < NG; ++g)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) { acc_r[g][v] = (R)0; acc_i[g][v] = (R)0; }
-    for (int c0 = piece.x; c0 < piece.y; c0 += kSQ) {
-        const int cn = min(kSQ, piece.y - c0);
-        // ---- staging, one source per lane.  Wave 0: phases and powers;
-        // wave 1: kernel values at the block's positions; waves 2-3: B operand.
-        if (wv == 0) {
-            const int q = lane;
-            double rj = q < cn ? T.r[c0 + q] : bx.C;
-            double x = rj - bx.C;
-            double sn, cs;
-            sincos_cw(bx.D * x, &sn, &cs);
-            double rs = q < cn ? 1.0 / sqrt(rj) : 0.0;
-            double br = cs * rs, bi = sn * rs;
-            double ri = q < cn ? bx.rho / rj : 0.0, rp = 1.0;
-            for (int e = 0; e < E; ++e) {
-                C2 u;
-                u.x = (R)(br * rp);
-                u.y = (R)(bi * rp);
-                L.u[q][e] = u;
-                rp *= ri;
+        for (int pi = 0; pi < kSP; ++pi) L.psi[q][pi] = (R)0;
+        if (q < cn) {
+            double x = T.r[c0 + q] - bx.C;
+            int lf = (int)ceil((x - bx.a1) * inv_h);
+            double f = ((double)lf * bx.h - x) * inv_h + 0.5 * kw;
+            R tb_[kEsN];
+            cheb_basis((R)(2.0 * f - 1.0), tb_);
+            const double *tc = ft + T.ft_off + kNFT;
+            for (int k = 0; k <= kw; ++k) {
+                int l = lf + k;
+                int pos = l >= 0 ? l : l + bx.Nf;
+                int pi = pos - it.start;
+                if (pi >= 0 && pi < it.count && l >= -bx.L && l <= bx.L)
+                    L.psi[q][pi] = es_tap(tc + k * kEsN, tb_);
             }
-        } else if (wv == 1) {
-            const int q = lane;
+        }
+    } else {
+        // B operand: 8 (source, column) entries per thread, loads issued 4 at a time
 #pragma unroll
-            for (int pi = 0; pi < kSP; ++pi) L.psi[q][pi] = (R)0;
-            if (q < cn) {
-                double x = T.r[c0 + q] - bx.C;
-                int lf = (int)ceil((x - bx.a1) * inv_h);
-                double f = ((double)lf * bx.h - x) * inv_h + 0.5 * kw;
-                R tb_[kEsN];
-                cheb_basis((R)(2.0 * f - 1.0), tb_);
-                const double *tc = ft + T.ft_off + kNFT;
-                for (int k = 0; k <= kw; ++k) {
-                    int l = lf + k;
-                    int pos = l >= 0 ? l : l + bx.Nf;
-                    int pi = pos - it.start;
-                    if (pi >= 0 && pi < it.count && l >= -bx.L && l <= bx.L)
-                        L.psi[q][pi] = es_tap(tc + k * kEsN, tb_);
-                }
-            }
-        } else {
-            // B operand: 8 (source, column) entries per thread, loads issued 4 at a time
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < 2; ++h) {
             double cv[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                int idx = tid - 128 + 128 * (4 * h + u);
+                int idx = ptid - 128 + 128 * (4 * h + u);
                 int q = idx & (kSQ - 1), j = idx >> 6;
                 int tt = j % tb;
                 int tc = it.col0 + tt;
@@ -867,7 +851,7 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                int idx = tid - 128 + 128 * (4 * h + u);
+                int idx = ptid - 128 + 128 * (4 * h + u);
                 int q = idx & (kSQ - 1), j = idx >> 6;
                 int e2 = j / tb;
                 double c = cv[u];
@@ -878,11 +862,72 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
                 }
                 L.bop[q][j] = (R)c;
             }
-            }
         }
+    }
+}
+
+// Two schedules of the same work.  SPLIT: a block of 8 waves,
+// waves 0-3 (consumers) each own 4 of the block's 16 positions and hold the
+// accumulators, waves 4-7 (producers) stage source chunks; chunks are
+// double-buffered in LDS (the consumers multiply chunk k out of buffer
+// k mod 2 while the producers stage chunk k + 1 into the other; one block
+// barrier per chunk), and the staging code's registers never coexist with
+// the accumulators (109 instead of 185 VGPRs at FP64, EMAX = 10); it pays
+// where each chunk carries enough products (FP64 grids with 3 or more
+// exponent groups: 5.98 -> 5.46 ms per headline step) and loses where
+// staging dominates (FP64 with 2 groups: 6.23 -> 6.86 ms per clustered
+// step).  Otherwise: a block of 4 waves that stage a chunk together, one
+// task each, then multiply it.
+template <typename R, int EMAX, int NG, bool SPLIT>
+__device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD &bx,
+                                                 const AsyItem &it, int2 piece,
+                                                 const SpreadRange *__restrict__ rg,
+                                                 typename Cplx<R>::T *__restrict__ grid,
+                                                 const double *__restrict__ ft,
+                                                 SpreadLds<R, EMAX> *L)
+{
+    typedef typename Cplx<R>::T C2;
+    typedef typename Mfma<R>::V V;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nchunk = (piece.y - piece.x + kSQ - 1) / kSQ;
+    if constexpr (SPLIT) {
+        const bool producer = wv >= 4;
+        if (producer && nchunk > 0)
+            spread_stage<R, EMAX>(T, bx, it, piece.x, min(kSQ, piece.y - piece.x), wv - 4,
+                                  tid - 256, ft, L[0]);
         __syncthreads();
+        if (producer) {
+            for (int k = 0; k < nchunk; ++k) {
+                const int c1 = piece.x + (k + 1) * kSQ;
+                if (k + 1 < nchunk)
+                    spread_stage<R, EMAX>(T, bx, it, c1, min(kSQ, piece.y - c1), wv - 4, tid - 256,
+                                          ft, L[(k + 1) & 1]);
+                __syncthreads();
+            }
+            return;
+        }
+    }
+    const int tb = it.tb, nsub = 16 / tb;
+    const int E = bx.E;
+    const int ka = lane >> 4, ia = lane & 15;
+    const int pa = ia >> 2, e1a = ia & 3;
+    const int jb = lane & 15;
+    const int2 trg = rg->tile[wv];
+    const int qa = max(trg.x, piece.x), qb = min(trg.y, piece.y);
+    V acc_r[NG], acc_i[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { acc_r[g][v] = (R)0; acc_i[g][v] = (R)0; }
+    for (int k = 0; k < nchunk; ++k) {
+        const int c0 = piece.x + k * kSQ, cn = min(kSQ, piece.y - c0);
         const int qlo = max(qa, c0) - c0, qhi = min(qb, c0 + cn) - c0;
-        spread_chunk<R, EMAX, NG>(L, qlo, qhi, 4 * wv + pa, E, nsub, ka, e1a, jb, acc_r, acc_i);
+        if constexpr (!SPLIT) {
+            spread_stage<R, EMAX>(T, bx, it, c0, cn, wv, tid, ft, L[0]);
+            __syncthreads();
+        }
+        spread_chunk<R, EMAX, NG>(L[SPLIT ? (k & 1) : 0], qlo, qhi, 4 * wv + pa, E, nsub, ka, e1a,
+                                  jb, acc_r, acc_i);
         __syncthreads();
     }
     // ---- result element (row i = (p, e1), column j = (t, e2)) of group g
@@ -912,33 +957,42 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
     }
 }
 
-template <typename R, int EMAX>
-__global__ __launch_bounds__(256) void k_asy_spread_mfma(const TfDesc *__restrict__ tfs,
-                                                         const AsyBoxD *__restrict__ boxes,
-                                                         const AsyItem *__restrict__ items,
-                                                         const SpreadRange *__restrict__ ranges,
-                                                         typename Cplx<R>::T *__restrict__ grid,
-                                                         const double *__restrict__ ft)
+// Block size and LDS bytes of the two schedules.
+template <bool SPLIT> struct SpreadSched {
+    static constexpr int threads = SPLIT ? 512 : 256;
+    template <typename R, int EMAX> static constexpr size_t lds()
+    {
+        return (SPLIT ? 2 : 1) * sizeof(SpreadLds<R, EMAX>);
+    }
+};
+
+template <typename R, int EMAX, bool SP>
+__global__ __launch_bounds__(SpreadSched<SP>::threads) void k_asy_spread_mfma(
+    const TfDesc *__restrict__ tfs, const AsyBoxD *__restrict__ boxes,
+    const AsyItem *__restrict__ items, const SpreadRange *__restrict__ ranges,
+    typename Cplx<R>::T *__restrict__ grid, const double *__restrict__ ft)
 {
-    __shared__ SpreadLds<R, EMAX> L;
+    // one or two SpreadLds buffers, dynamic (above 64 KiB for the widest EMAX)
+    extern __shared__ unsigned char s_spread[];
+    SpreadLds<R, EMAX> *L = (SpreadLds<R, EMAX> *)s_spread;
     const AsyItem it = items[blockIdx.x];
     const int2 piece = ranges[blockIdx.x].piece;
-    const int2 trg = ranges[blockIdx.x].tile[threadIdx.x >> 6];
     const AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
     const int nsub = 16 / it.tb;
     const int G = (bx.E + 4 * nsub - 1) / (4 * nsub);
+    const SpreadRange *rg = ranges + blockIdx.x;
     // the group count is uniform per block: one branch-free body per count
     constexpr int GMAX = (EMAX + 3) / 4;
-    if (G <= 1) spread_mfma_body<R, EMAX, 1>(T, bx, it, piece, trg, grid, ft, L);
-    else if (G == 2 || GMAX == 2) spread_mfma_body<R, EMAX, 2>(T, bx, it, piece, trg, grid, ft, L);
+    if (G <= 1) spread_mfma_body<R, EMAX, 1, SP>(T, bx, it, piece, rg, grid, ft, L);
+    else if (G == 2 || GMAX == 2) spread_mfma_body<R, EMAX, 2, SP>(T, bx, it, piece, rg, grid, ft, L);
     else if constexpr (GMAX >= 3) {
-        if (G == 3 || GMAX == 3) spread_mfma_body<R, EMAX, 3>(T, bx, it, piece, trg, grid, ft, L);
+        if (G == 3 || GMAX == 3) spread_mfma_body<R, EMAX, 3, SP>(T, bx, it, piece, rg, grid, ft, L);
         else if constexpr (GMAX >= 4) {
-            if (G == 4 || GMAX == 4) spread_mfma_body<R, EMAX, 4>(T, bx, it, piece, trg, grid, ft, L);
+            if (G == 4 || GMAX == 4) spread_mfma_body<R, EMAX, 4, SP>(T, bx, it, piece, rg, grid, ft, L);
             else if constexpr (GMAX >= 6) {
-                if (G == 5) spread_mfma_body<R, EMAX, 5>(T, bx, it, piece, trg, grid, ft, L);
-                else spread_mfma_body<R, EMAX, 6>(T, bx, it, piece, trg, grid, ft, L);
+                if (G == 5) spread_mfma_body<R, EMAX, 5, SP>(T, bx, it, piece, rg, grid, ft, L);
+                else spread_mfma_body<R, EMAX, 6, SP>(T, bx, it, piece, rg, grid, ft, L);
             }
         }
     }

@@ -34,6 +34,8 @@ ALTERNATIVES = [
     {"own_fft": 0},                 # rocFFT for every length
     {"dense_split": 2048, "spread_split": 1024},
     {"dense_split": 8192, "spread_split": 4096},
+    {"spread_sched": 0},            # matrix-core spread: unified schedule
+    {"spread_sched": 1},            # matrix-core spread: split schedule
 ]
 
 
@@ -48,7 +50,7 @@ def test_every_knob_path_meets_tolerance(device):
     for alt in ALTERNATIVES:
         say(f"nu=3 M=N=1e5 B=4 tol=1e-8 knobs {alt or 'default'}")
         p.set_knobs(**{**{"dense_cell": 4.0, "dense_split": 4096, "spread_split": 2048,
-                          "interp_mfma": 1, "own_fft": 1}, **alt})
+                          "interp_mfma": 1, "own_fft": 1, "spread_sched": -1}, **alt})
         st = p.stats(4)
         if alt.get("dense_cell") == 1e300:
             assert st["spread_sc_mfma"] == 0
