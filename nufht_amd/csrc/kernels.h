@@ -1023,6 +1023,7 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
     typedef typename Mfma<R>::V V;
     __shared__ double s_s[16], s_fr[16], s_fi[16], s_b0[16];
     __shared__ double s_P[16][EMAX];
+    __shared__ double s_out[16][17];   // result tile, transposed for the atomic adds
     __shared__ int s_m0[16];
     AsyItem it = items[blockIdx.x];
     AsyBoxD bx = boxes[it.box];
@@ -1055,14 +1056,25 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
     __syncthreads();
     int nwin = s_m0[it.count - 1] + kw - mw0;
     int nsteps = min(4, (nwin + 3) >> 2);
-    // A operand (phi) for each K step, reused by every exponent group
+    // FP32 grids: the window starts at an even position (it still covers
+    // every support: the tile test leaves one position of slack) and each
+    // lane loads two adjacent complex values with one 16-byte load; K step
+    // st then holds window positions 8 (st >> 1) + 2 ka + (st & 1).  Pairs
+    // never straddle the end of the grid (Nf is even).  Positions outside a
+    // target's support get phi = 0 from the kernel itself.
+    constexpr bool PAIR = sizeof(C2) == 8;
     const int ia = lane & 15, ka = lane >> 4;
+    const int mwb = PAIR ? (mw0 & ~1) : mw0;
+    auto kpos = [&](int st) {
+        return PAIR ? mwb + 8 * (st >> 1) + 2 * ka + (st & 1) : mwb + 4 * st + ka;
+    };
+    // A operand (phi) for each K step, reused by every exponent group
     R phiA[4];
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
-        int m = mw0 + 4 * st + ka;
+        int m = kpos(st);
         double u = (s_s[ia] - (double)m * bx.delta) / bx.a2;
-        phiA[st] = (st < nsteps && ia < it.count) ? (R)es_kernel(u, T.beta) : (R)0;
+        phiA[st] = ((PAIR || st < nsteps) && ia < it.count) ? (R)es_kernel(u, T.beta) : (R)0;
     }
     const int jb = lane & 15, tbk = jb % tb, e2b = jb / tb;
     const int tcol = it.col0 + tbk;
@@ -1070,7 +1082,7 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
     int moff[4];
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
-        int m = mw0 + 4 * st + ka;
+        int m = PAIR ? kpos(st & 2) : kpos(st);     // PAIR: steps 2s, 2s+1 share one load
         moff[st] = m < 0 ? m + bx.Nf : (m >= bx.Nf ? m - bx.Nf : m);
     }
     double Sr[4] = {0.0, 0.0, 0.0, 0.0}, Si[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1081,8 +1093,19 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
         // the step's 4 grid values: loads at valid addresses, issued together
         const C2 *row = grid + bx.goff + (long long)((eok ? eb : 0) * T.B + (colok ? tcol : 0)) * bx.Nf;
         C2 b[4];
+        if constexpr (PAIR) {
 #pragma unroll
-        for (int st = 0; st < 4; ++st) b[st] = row[moff[st]];
+            for (int h = 0; h < 2; ++h) {
+                const float4 v = *(const float4 *)(row + moff[2 * h]);
+                b[2 * h].x = v.x;
+                b[2 * h].y = v.y;
+                b[2 * h + 1].x = v.z;
+                b[2 * h + 1].y = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int st = 0; st < 4; ++st) b[st] = row[moff[st]];
+        }
         V dr0, di0, dr1, di1;
 #pragma unroll
         for (int v = 0; v < 4; ++v) { dr0[v] = (R)0; di0[v] = (R)0; dr1[v] = (R)0; di1[v] = (R)0; }
@@ -1117,12 +1140,17 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
     for (int off = tb; off < 16; off <<= 1)
 #pragma unroll
         for (int v = 0; v < 4; ++v) out[v] += __shfl_xor(out[v], off, 64);
-    if (e2b == 0 && colok) {
+    // through LDS, so that each atomic instruction adds 16 consecutive rows
+    // of 4 columns (4 whole 128-byte lines of g) instead of 4 rows of 16
+    if (e2b == 0) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            int i = Mfma<R>::row(lane, v);
-            if (i < it.count) atomicAdd(&T.gs[(long long)tcol * T.ld_gs + bx.i0 + it.start + i], out[v]);
-        }
+        for (int v = 0; v < 4; ++v) s_out[Mfma<R>::row(lane, v)][jb] = out[v];
+    }
+    __syncthreads();
+    for (int idx = lane; idx < 16 * tb; idx += 64) {
+        const int c = idx >> 4, i = idx & 15;
+        if (i < it.count && it.col0 + c < T.B)
+            atomicAdd(&T.gs[(long long)(it.col0 + c) * T.ld_gs + bx.i0 + it.start + i], s_out[i][c]);
     }
 }
 
