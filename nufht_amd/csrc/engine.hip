@@ -941,8 +941,11 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 int m0l = (int)std::ceil((sl - A.a2) / A.delta);
                 if (P->knobs.interp_mfma && m0l - m0f + T.kw + 2 <= 16) {
                     flush(s);
-                    P->aimf.push_back({box, s, cnt, c0, tb, m0f - 1});
-                    P->stats.interp_tc_mfma += (long long)cnt * chans;
+                    // one item per tile covers every column chunk
+                    if (c0 == 0) {
+                        P->aimf.push_back({box, s, cnt, 0, tb, m0f - 1});
+                        P->stats.interp_tc_mfma += (long long)cnt * A.E * T.B;
+                    }
                 } else if (run0 < 0) {
                     run0 = s;
                 }

@@ -1077,16 +1077,18 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
         phiA[st] = ((PAIR || st < nsteps) && ia < it.count) ? (R)es_kernel(u, T.beta) : (R)0;
     }
     const int jb = lane & 15, tbk = jb % tb, e2b = jb / tb;
-    const int tcol = it.col0 + tbk;
-    const bool colok = tcol < T.B;
     int moff[4];
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
         int m = PAIR ? kpos(st & 2) : kpos(st);     // PAIR: steps 2s, 2s+1 share one load
         moff[st] = m < 0 ? m + bx.Nf : (m >= bx.Nf ? m - bx.Nf : m);
     }
-    double Sr[4] = {0.0, 0.0, 0.0, 0.0}, Si[4] = {0.0, 0.0, 0.0, 0.0};
     const int G = (E + nsub - 1) / nsub;
+    // every column chunk of the tile, with its phi and per-target factors
+    for (int cb = it.col0; cb < T.B; cb += tb) {
+    const int tcol = cb + tbk;
+    const bool colok = tcol < T.B;
+    double Sr[4] = {0.0, 0.0, 0.0, 0.0}, Si[4] = {0.0, 0.0, 0.0, 0.0};
     for (int g = 0; g < G; ++g) {
         const int eb = g * nsub + e2b;
         const bool eok = colok && eb < E;
@@ -1149,8 +1151,10 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
     __syncthreads();
     for (int idx = lane; idx < 16 * tb; idx += 64) {
         const int c = idx >> 4, i = idx & 15;
-        if (i < it.count && it.col0 + c < T.B)
-            atomicAdd(&T.gs[(long long)(it.col0 + c) * T.ld_gs + bx.i0 + it.start + i], s_out[i][c]);
+        if (i < it.count && cb + c < T.B)
+            atomicAdd(&T.gs[(long long)(cb + c) * T.ld_gs + bx.i0 + it.start + i], s_out[i][c]);
+    }
+    __syncthreads();
     }
 }
 
