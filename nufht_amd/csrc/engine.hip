@@ -386,6 +386,8 @@ struct Program {
     ZeroItem *d_zero = nullptr;
     std::vector<DirItem> dir;
     DirItem *d_dir = nullptr;
+    std::vector<int2> dir_rng;        // source ranges {j0, count} of the direct items
+    int2 *d_dir_rng = nullptr;
     std::vector<LocBoxD> lbox;
     LocBoxD *d_lbox = nullptr;
     std::vector<LocItem> lproj, leval;
@@ -466,7 +468,8 @@ static int upload(T **dptr, const std::vector<T> &v)
 static void free_program(Program *P)
 {
     if (!P) return;
-    hipFree(P->d_tfs); hipFree(P->d_zero); hipFree(P->d_dir); hipFree(P->d_lbox);
+    hipFree(P->d_tfs); hipFree(P->d_zero); hipFree(P->d_dir); hipFree(P->d_dir_rng);
+    hipFree(P->d_lbox);
     hipFree(P->d_lproj); hipFree(P->d_leval); hipFree(P->d_coef); hipFree(P->d_abox);
     hipFree(P->d_aspread); hipFree(P->d_ainterp); hipFree(P->d_grid); hipFree(P->d_ft);
     hipFree(P->d_grid2); hipFree(P->d_fftw); hipFree(P->d_fft_chinfo);
@@ -645,6 +648,70 @@ static void add_chunks_2d(std::vector<LocItem> &v, int box, int count, int chunk
         for (int s = 0; s < count; s += chunk) v.push_back({box, s, std::min(chunk, count - s), c0});
 }
 
+// Direct-sum items.  A target row is covered by several direct boxes (about
+// 4 at the headline workload), each with few sources per row; one item per
+// (row chunk, source slice, column chunk) of a row segment whose covering
+// boxes are the same sums the sources of all of them, so that each output
+// element receives one atomic add per slice instead of one per box.
+constexpr int kDirSlice = 8192;      // sources per item (slices of long strips)
+
+static void add_dir_items(Program *P, int tf, int B, const std::vector<Box> &dirb)
+{
+    std::vector<int> cuts;
+    for (auto &b : dirb)
+        if (b.i1 >= b.i0 && b.j1 >= b.j0) {
+            cuts.push_back(b.i0);
+            cuts.push_back(b.i1 + 1);
+        }
+    std::sort(cuts.begin(), cuts.end());
+    cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+    for (size_t q = 0; q + 1 < cuts.size(); ++q) {
+        const int a = cuts[q], e = cuts[q + 1], ni = e - a;
+        // the segment's source ranges (boxes tile the plane: disjoint in j)
+        std::vector<int2> rg;
+        for (auto &b : dirb)
+            if (b.i1 >= b.i0 && b.j1 >= b.j0 && b.i0 <= a && b.i1 >= e - 1)
+                rg.push_back(make_int2(b.j0, b.j1 - b.j0 + 1));
+        if (rg.empty()) continue;
+        std::sort(rg.begin(), rg.end(), [](const int2 &x, const int2 &y) { return x.x < y.x; });
+        long long S = 0;
+        for (auto &r : rg) S += r.y;
+        // lanes per row: rows with few sources get one lane each (no
+        // reduction); blocks of few rows cover all of them at once
+        int tpr;
+        if (ni >= 64) {
+            tpr = S <= 32 ? 1 : (S <= 64 ? 2 : 4);
+        } else {
+            int p2 = 1;
+            while (p2 < ni) p2 <<= 1;
+            tpr = std::min(64, std::max(4, 256 / p2));
+        }
+        const int rows = 256 / tpr;
+        // source slices of at most kDirSlice sources, as lists of ranges
+        std::vector<std::pair<int, int>> slices;   // (offset in dir_rng, count)
+        {
+            size_t k = 0;
+            int off = 0;
+            while (k < rg.size()) {
+                int roff = (int)P->dir_rng.size(), taken = 0;
+                while (k < rg.size() && taken < kDirSlice) {
+                    int n = std::min(rg[k].y - off, kDirSlice - taken);
+                    P->dir_rng.push_back(make_int2(rg[k].x + off, n));
+                    taken += n;
+                    off += n;
+                    if (off == rg[k].y) { ++k; off = 0; }
+                }
+                slices.push_back({roff, (int)P->dir_rng.size() - roff});
+            }
+        }
+        for (int c0 = 0; c0 < B; c0 += kDirBM)
+            for (int i = 0; i < ni; i += rows)
+                for (auto &sl : slices)
+                    P->dir.push_back({tf, a + i, std::min(rows, ni - i), sl.first, sl.second, tpr,
+                                      c0, 0});
+    }
+}
+
 static int build_program(Program *P, const std::vector<TfInput> &in)
 {
     static std::once_flag fft_once;
@@ -717,25 +784,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         for (auto &b : dirb) {
             int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1;
             if (ni <= 0 || nj <= 0) continue;
-            // lanes per row: rows with few sources get one lane each (no
-            // reduction); blocks of few rows cover all of them at once
-            int tpr;
-            if (ni >= 64) {
-                tpr = nj <= 32 ? 1 : (nj <= 64 ? 2 : 4);
-            } else {
-                int p2 = 1;
-                while (p2 < ni) p2 <<= 1;
-                tpr = std::min(64, std::max(4, 256 / p2));
-            }
-            int rows = 256 / tpr;
-            for (int c0 = 0; c0 < B; c0 += kDirBM)
-                for (int i = 0; i < ni; i += rows)
-                    for (int j = 0; j < nj; j += kDirTJ)
-                        P->dir.push_back({tf, b.i0 + i, std::min(rows, ni - i), b.j0 + j,
-                                          std::min(kDirTJ, nj - j), tpr, c0, 0});
             P->stats.pairs_dir += (long long)ni * nj;
             P->stats.n_dir++;
         }
+        add_dir_items(P, tf, B, dirb);
         if (!locb.empty()) {
             kloc_max = std::max(kloc_max, T.K_loc);
             slots_max = std::max(slots_max, loc_slots(T.order, T.half, T.K_loc));
@@ -999,6 +1051,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     // uploads
     RC_TRY(upload(&P->d_zero, P->zero));
     RC_TRY(upload(&P->d_dir, P->dir));
+    RC_TRY(upload(&P->d_dir_rng, P->dir_rng));
     RC_TRY(upload(&P->d_lbox, P->lbox));
     RC_TRY(upload(&P->d_lproj, P->lproj));
     RC_TRY(upload(&P->d_leval, P->leval));
@@ -1231,7 +1284,7 @@ static int run_program(Program *P, hipStream_t st)
     if (!P->dir.empty()) {
         StageTimer tm(ST_DIR, st);
         hipLaunchKernelGGL(k_dir, dim3((unsigned)P->dir.size()), dim3(256), 0, st, P->d_tfs,
-                           P->d_dir, P->d_jtab);
+                           P->d_dir, P->d_dir_rng, P->d_jtab);
     }
     if (!P->aspread.empty() || !P->adense.empty()) {
         if (P->prec == 2) launch_asy_e<float>(P, st);

@@ -41,7 +41,9 @@ struct TfDesc {
     double a[kMaxA];       // Hankel coefficients a_k(|nu|)
 };
 
-struct DirItem { int tf, i0, ni, j0, nj, tpr, col0, pad; };
+// rows [i0, i0 + ni) of transform tf, columns [col0, col0 + 16), sources in
+// the ranges rng[roff .. roff + rcnt) ({j0, count} each)
+struct DirItem { int tf, i0, ni, roff, rcnt, tpr, col0, pad; };
 struct LocBoxD { int tf, i0, ni, j0, nj, coef_off; double r_end; };
 struct LocItem { int box, start, count, col0; };
 struct AsyBoxD {
@@ -226,6 +228,7 @@ __global__ void k_jtab_build(int order, int half, int nint, double *__restrict__
 
 __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
                                              const DirItem *__restrict__ items,
+                                             const int2 *__restrict__ rng,
                                              const double *__restrict__ jtab)
 {
     constexpr int LDC = kDirBM + 1;                 // padded row: no bank conflicts
@@ -234,19 +237,6 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
     DirItem it = items[blockIdx.x];
     const TfDesc &T = tfs[it.tf];
     int nb = min(kDirBM, T.B - it.col0);
-    {
-        // one source per thread (nj <= kDirTJ = blockDim), its columns loaded together
-        const int j = threadIdx.x;
-        const bool ok = j < it.nj;
-        double cv[kDirBM];
-#pragma unroll
-        for (int t = 0; t < kDirBM; ++t)
-            cv[t] = (ok && t < nb) ? T.cs[(long long)(it.col0 + t) * T.ld_cs + it.j0 + j] : 0.0;
-        s_r[j] = ok ? T.r[it.j0 + j] : 0.0;
-#pragma unroll
-        for (int t = 0; t < kDirBM; ++t) s_c[j * LDC + t] = cv[t];
-    }
-    __syncthreads();
     int tpr = it.tpr;
     int row = threadIdx.x / tpr, lane = threadIdx.x % tpr;
     bool active = row < it.ni;
@@ -255,19 +245,39 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
     for (int t = 0; t < kDirBM; ++t) acc[t] = 0.0;
     const double *tab = jtab + T.jtab_off;
     double xmax = (double)T.jtab_n;
-    if (active) {
-        double w = T.w[it.i0 + row];
-        for (int j = lane; j < it.nj; j += tpr) {
-            double x = w * s_r[j];
-            double J;
-            if (x >= 0.0 && x < xmax) {
-                J = jtab_eval(tab, x);
-                if (T.half) J *= sqrt(x);
-            } else {
-                J = besselj_wr(T.order, T.half, w, s_r[j]);
-            }
+    const double w = active ? T.w[it.i0 + row] : 0.0;
+    for (int q = 0; q < it.rcnt; ++q) {
+        const int2 rq = rng[it.roff + q];
+        for (int c = 0; c < rq.y; c += kDirTJ) {
+            const int nj = min(kDirTJ, rq.y - c), j0 = rq.x + c;
+            __syncthreads();   // the previous chunk's readers are done
+            {
+                // one source per thread (nj <= kDirTJ = blockDim), its columns loaded together
+                const int j = threadIdx.x;
+                const bool ok = j < nj;
+                double cv[kDirBM];
 #pragma unroll
-            for (int t = 0; t < kDirBM; ++t) acc[t] = fma(J, s_c[j * LDC + t], acc[t]);
+                for (int t = 0; t < kDirBM; ++t)
+                    cv[t] = (ok && t < nb) ? T.cs[(long long)(it.col0 + t) * T.ld_cs + j0 + j] : 0.0;
+                s_r[j] = ok ? T.r[j0 + j] : 0.0;
+#pragma unroll
+                for (int t = 0; t < kDirBM; ++t) s_c[j * LDC + t] = cv[t];
+            }
+            __syncthreads();
+            if (active) {
+                for (int j = lane; j < nj; j += tpr) {
+                    double x = w * s_r[j];
+                    double J;
+                    if (x >= 0.0 && x < xmax) {
+                        J = jtab_eval(tab, x);
+                        if (T.half) J *= sqrt(x);
+                    } else {
+                        J = besselj_wr(T.order, T.half, w, s_r[j]);
+                    }
+#pragma unroll
+                    for (int t = 0; t < kDirBM; ++t) acc[t] = fma(J, s_c[j * LDC + t], acc[t]);
+                }
+            }
         }
     }
     for (int off = tpr >> 1; off > 0; off >>= 1) {
