@@ -390,8 +390,15 @@ struct Program {
     int2 *d_dir_rng = nullptr;
     std::vector<LocBoxD> lbox;
     LocBoxD *d_lbox = nullptr;
-    std::vector<LocItem> lproj, leval;
-    LocItem *d_lproj = nullptr, *d_leval = nullptr;
+    std::vector<LocItem> lproj;
+    LocItem *d_lproj = nullptr;
+    std::vector<LocItem> leval1;      // targets covered by one local box
+    LocItem *d_leval1 = nullptr;
+    std::vector<LocEvalItem> leval;   // targets covered by several
+    LocEvalItem *d_leval = nullptr;
+    std::vector<int> lev_boxes;       // box lists of the evaluation items
+    int *d_lev_boxes = nullptr;
+    int loc_bmax = 16;                // columns per evaluation item (16 or 32)
     double *d_coef = nullptr;
     long long ncoef = 0;
     int loc_ch = 1024, loc_tpb = 256, loc_nlt = 1;
@@ -470,7 +477,9 @@ static void free_program(Program *P)
     if (!P) return;
     hipFree(P->d_tfs); hipFree(P->d_zero); hipFree(P->d_dir); hipFree(P->d_dir_rng);
     hipFree(P->d_lbox);
-    hipFree(P->d_lproj); hipFree(P->d_leval); hipFree(P->d_coef); hipFree(P->d_abox);
+    hipFree(P->d_lproj); hipFree(P->d_leval); hipFree(P->d_leval1); hipFree(P->d_lev_boxes);
+    hipFree(P->d_coef);
+    hipFree(P->d_abox);
     hipFree(P->d_aspread); hipFree(P->d_ainterp); hipFree(P->d_grid); hipFree(P->d_ft);
     hipFree(P->d_grid2); hipFree(P->d_fftw); hipFree(P->d_fft_chinfo);
     hipFree(P->d_azero); hipFree(P->d_arange); hipFree(P->d_adense); hipFree(P->d_adrange);
@@ -646,6 +655,61 @@ static void add_chunks_2d(std::vector<LocItem> &v, int box, int count, int chunk
 {
     for (int c0 = 0; c0 < B; c0 += cols)
         for (int s = 0; s < count; s += chunk) v.push_back({box, s, std::min(chunk, count - s), c0});
+}
+
+// Local-expansion evaluation items.  Where the atomic adds saved are many,
+// B (boxes per covered target - 1) >= 40 (the clustered workload: 32 x
+// 1.6-1.8, where this form measured 1.12 -> 0.75 ms per step; the headline
+// workload has 16 x 1.1 and measured 0.19 -> 0.34 ms), per
+// segment of target rows covered by the same boxes, chunks of loc_tpb
+// targets x loc_bmax columns that loop over all of those boxes (k_loc_eval:
+// one atomic add per output element per item).  Otherwise per box, chunks of
+// loc_tpb targets x all columns (k_loc_eval1: fewer registers; segments cut
+// at every box boundary would leave many small items).
+static void add_loc_eval_items(Program *P)
+{
+    std::map<int, std::vector<int>> bytf;
+    for (size_t b = 0; b < P->lbox.size(); ++b) bytf[P->lbox[b].tf].push_back((int)b);
+    for (auto &kv : bytf) {
+        const int tf = kv.first, B = P->tfs[tf].B;
+        const std::vector<int> &bl = kv.second;
+        std::vector<int> cuts;
+        long long cover = 0;
+        for (int b : bl) {
+            cuts.push_back(P->lbox[b].i0);
+            cuts.push_back(P->lbox[b].i0 + P->lbox[b].ni);
+            cover += P->lbox[b].ni;
+        }
+        std::sort(cuts.begin(), cuts.end());
+        cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+        // targets covered at least once
+        long long covered = 0;
+        for (size_t q = 0; q + 1 < cuts.size(); ++q)
+            for (int b : bl)
+                if (P->lbox[b].i0 <= cuts[q] && P->lbox[b].i0 + P->lbox[b].ni >= cuts[q + 1]) {
+                    covered += cuts[q + 1] - cuts[q];
+                    break;
+                }
+        // atomics saved per target ~ B (boxes per covered target - 1)
+        if (covered == 0 || (double)B * ((double)cover / (double)covered - 1.0) < 40.0) {
+            for (int b : bl)
+                for (int s = 0; s < P->lbox[b].ni; s += P->loc_tpb)
+                    P->leval1.push_back({b, s, std::min(P->loc_tpb, P->lbox[b].ni - s), 0});
+            continue;
+        }
+        for (size_t q = 0; q + 1 < cuts.size(); ++q) {
+            const int a = cuts[q], e = cuts[q + 1];
+            const int roff = (int)P->lev_boxes.size();
+            for (int b : bl)
+                if (P->lbox[b].i0 <= a && P->lbox[b].i0 + P->lbox[b].ni >= e)
+                    P->lev_boxes.push_back(b);
+            const int rcnt = (int)P->lev_boxes.size() - roff;
+            if (!rcnt) continue;
+            for (int c0 = 0; c0 < B; c0 += P->loc_bmax)
+                for (int i = a; i < e; i += P->loc_tpb)
+                    P->leval.push_back({tf, i, std::min(P->loc_tpb, e - i), roff, rcnt, c0});
+        }
+    }
 }
 
 // Direct-sum items.  A target row is covered by several direct boxes (about
@@ -1031,15 +1095,23 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
             nufht_set_error("local expansion scratch of %d words per target exceeds LDS", slots_max);
             return -540;
         }
-        if (P->loc_lds_e > 64 * 1024)
-            HIP_TRY(hipFuncSetAttribute((const void *)k_loc_eval,
+        if (P->loc_lds_e > 64 * 1024) {
+            HIP_TRY(hipFuncSetAttribute((const void *)k_loc_eval1,
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)P->loc_lds_e));
+            HIP_TRY(hipFuncSetAttribute((const void *)k_loc_eval<16>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)P->loc_lds_e));
+            HIP_TRY(hipFuncSetAttribute((const void *)k_loc_eval<32>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)P->loc_lds_e));
+        }
         for (size_t b = 0; b < P->lbox.size(); ++b) {
             const TfDesc &T = P->tfs[P->lbox[b].tf];
             add_chunks_2d(P->lproj, (int)b, P->lbox[b].nj, P->loc_ch, T.B, kLocBM);
-            add_chunks_2d(P->leval, (int)b, P->lbox[b].ni, P->loc_tpb, T.B, T.B);
+            if (T.B > 16) P->loc_bmax = 32;
         }
+        add_loc_eval_items(P);
     }
     if (!P->tfs.empty()) {
         P->stats.kernel_width = P->tfs[0].kw;
@@ -1055,6 +1127,8 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     RC_TRY(upload(&P->d_lbox, P->lbox));
     RC_TRY(upload(&P->d_lproj, P->lproj));
     RC_TRY(upload(&P->d_leval, P->leval));
+    RC_TRY(upload(&P->d_leval1, P->leval1));
+    RC_TRY(upload(&P->d_lev_boxes, P->lev_boxes));
     RC_TRY(upload(&P->d_abox, P->abox));
     RC_TRY(upload(&P->d_aspread, P->aspread));
     RC_TRY(upload(&P->d_ainterp, P->ainterp));
@@ -1276,9 +1350,19 @@ static int run_program(Program *P, hipStream_t st)
         }
         {
             StageTimer tm(ST_LOCE, st);
-            hipLaunchKernelGGL(k_loc_eval, dim3((unsigned)P->leval.size()), dim3(P->loc_tpb),
-                               P->loc_lds_e, st, P->d_tfs, P->d_lbox, P->d_leval, P->d_coef,
-                               P->d_cheb);
+            if (!P->leval1.empty())
+                hipLaunchKernelGGL(k_loc_eval1, dim3((unsigned)P->leval1.size()), dim3(P->loc_tpb),
+                                   P->loc_lds_e, st, P->d_tfs, P->d_lbox, P->d_leval1, P->d_coef,
+                                   P->d_cheb);
+            if (P->leval.empty()) {
+            } else if (P->loc_bmax == 32)
+                hipLaunchKernelGGL(k_loc_eval<32>, dim3((unsigned)P->leval.size()),
+                                   dim3(P->loc_tpb), P->loc_lds_e, st, P->d_tfs, P->d_lbox,
+                                   P->d_leval, P->d_lev_boxes, P->d_coef, P->d_cheb);
+            else
+                hipLaunchKernelGGL(k_loc_eval<16>, dim3((unsigned)P->leval.size()),
+                                   dim3(P->loc_tpb), P->loc_lds_e, st, P->d_tfs, P->d_lbox,
+                                   P->d_leval, P->d_lev_boxes, P->d_coef, P->d_cheb);
         }
     }
     if (!P->dir.empty()) {

@@ -46,6 +46,9 @@ struct TfDesc {
 struct DirItem { int tf, i0, ni, roff, rcnt, tpr, col0, pad; };
 struct LocBoxD { int tf, i0, ni, j0, nj, coef_off; double r_end; };
 struct LocItem { int box, start, count, col0; };
+// targets [i0, i0 + count) of transform tf, columns [col0, col0 + BMAX), the
+// local boxes boxlist[roff .. roff + rcnt) that cover them
+struct LocEvalItem { int tf, i0, count, roff, rcnt, col0; };
 struct AsyBoxD {
     int tf, i0, ni, j0, nj, E, L, Nf;
     long long goff;
@@ -630,16 +633,22 @@ __global__ __launch_bounds__(256) void k_loc_proj(const TfDesc *__restrict__ tfs
 // l > l0 with l - l0 odd); half-integer orders sample j_n(w r_end x_q) at
 // the K+1 positive Chebyshev nodes and map them to coefficients with the
 // precomputed matrix cheb[].  Then g[i,t] += sum_l b_l coef[l][t] for every
-// column t.  One thread per target; the per-target scratch lives in LDS as
+// column t.  One thread per target.  k_loc_eval1: one local box per item;
+// the sums go out four columns at a time.  k_loc_eval: every box that
+// covers the item's targets (about 2.6 at the clustered workload), looped
+// over with up to BMAX column sums in registers, so that each output element
+// gets one atomic add per item instead of one per box (the registers cost
+// occupancy, so transforms whose targets mostly meet one box keep the first
+// form).  The per-target scratch lives in LDS as
 // [slot][thread] (slots = loc_slots()), so lanes touch consecutive words;
-// the coefficients are the same for the whole block (one box) and are read
-// as uniform loads, four columns at a time.
+// a box's coefficients are the same for the whole block and are read as
+// uniform loads.
 __host__ __device__ inline int loc_slots(int order, int half, int K)
 {
     return half ? 2 * (K + 1) : (order >> 1) + K + (order & 1) + 1;
 }
 
-__global__ void k_loc_eval(const TfDesc *__restrict__ tfs, const LocBoxD *__restrict__ boxes,
+__global__ __launch_bounds__(256) void k_loc_eval1(const TfDesc *__restrict__ tfs, const LocBoxD *__restrict__ boxes,
                            const LocItem *__restrict__ items, const double *__restrict__ coef,
                            const double *__restrict__ cheb)
 {
@@ -729,6 +738,97 @@ __global__ void k_loc_eval(const TfDesc *__restrict__ tfs, const LocBoxD *__rest
         if (nt > 2) atomicAdd(g + 2 * (long long)T.ld_gs, sc * a2);
         if (nt > 3) atomicAdd(g + 3 * (long long)T.ld_gs, sc * a3);
     }
+}
+
+template <int BMAX>
+__global__ __launch_bounds__(256) void k_loc_eval(const TfDesc *__restrict__ tfs, const LocBoxD *__restrict__ boxes,
+                           const LocEvalItem *__restrict__ items,
+                           const int *__restrict__ boxlist, const double *__restrict__ coef,
+                           const double *__restrict__ cheb)
+{
+    extern __shared__ double sm[];
+    const LocEvalItem it = items[blockIdx.x];
+    const TfDesc &T = tfs[it.tf];
+    const int K = T.K_loc;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    if (tid >= it.count) return;
+    double *S = sm + tid;   // slot k at S[k * nthr]
+    const int i = it.i0 + tid;
+    const double w = T.w[i];
+    const int B = T.B, nb = min(BMAX, B - it.col0);
+    double acc[BMAX];
+#pragma unroll
+    for (int t = 0; t < BMAX; ++t) acc[t] = 0.0;
+    for (int q = 0; q < it.rcnt; ++q) {
+        const LocBoxD bx = boxes[boxlist[it.roff + q]];
+        int boff;               // slot of b_0 (b_l at slot boff + l)
+        if (!T.half) {
+            const int l0 = T.order >> 1, odd = T.order & 1;
+            const int maxo = l0 + K + odd;
+            const double z = w * bx.r_end * 0.5;
+            if (z == 0.0) {
+                for (int k = 0; k <= maxo; ++k) S[k * nthr] = (k == 0) ? 1.0 : 0.0;
+            } else {
+                double m = (double)(maxo > (int)z ? maxo : (int)z);
+                int N = (int)(m + 20.0 + 4.0 * sqrt(m + 1.0) + 6.0 * cbrt(m + 1.0));
+                N += N & 1;
+                double jp1 = 0.0, j = 1e-280, sum = 0.0, tx = 2.0 / z;
+                for (int k = N; k >= 1; --k) {
+                    double jm1 = (double)k * tx * j - jp1;
+                    jp1 = j;
+                    j = jm1;
+                    int km1 = k - 1;
+                    if (km1 <= maxo) S[km1 * nthr] = j;
+                    if (km1 > 0 && !(km1 & 1)) sum += 2.0 * j;
+                    if (fabs(j) > 1e200) {
+                        j *= 1e-200; jp1 *= 1e-200; sum *= 1e-200;
+                        for (int qq = km1; qq <= maxo; ++qq) S[qq * nthr] *= 1e-200;
+                    }
+                }
+                double inv = 1.0 / (sum + j);
+                for (int k = 0; k <= maxo; ++k) S[k * nthr] *= inv;
+            }
+            // b_l replaces J_{l0+odd+l}; taken from the top down, every J still
+            // needed sits below the slots already replaced
+            for (int l = K; l >= 0; --l) {
+                int lo = l0 - l;
+                lo = lo < 0 ? -lo : lo;
+                double v = S[(l0 + odd + l) * nthr] * S[lo * nthr];
+                if (l > l0 && ((l - l0) & 1)) v = -v;
+                S[(l0 + odd + l) * nthr] = v;
+            }
+            boff = l0 + odd;
+        } else {
+            // f_q = j_n(z x_q), x_q = cos(pi (q + 1/2) / (2K + 2)), q = 0..K, at
+            // slots K+1+q; b_l at slots l
+            const double z = w * bx.r_end;
+            for (int qq = 0; qq <= K; ++qq) {
+                double xq = cos(kPi * ((double)qq + 0.5) / (double)(2 * K + 2));
+                double hi = z * xq, lo = fma(z, xq, -hi);
+                S[(K + 1 + qq) * nthr] = sph_jn(T.order, hi, lo);
+            }
+            const double *M = cheb + T.cheb_off;
+            for (int l = 0; l <= K; ++l) {
+                double a = 0.0;
+                for (int qq = 0; qq <= K; ++qq) a = fma(M[l * (K + 1) + qq], S[(K + 1 + qq) * nthr], a);
+                S[l * nthr] = a;
+            }
+            boff = 0;
+        }
+        const double *cf = coef + bx.coef_off + it.col0;
+        for (int l = 0; l <= K; ++l) {
+            const double bl = S[(boff + l) * nthr];
+            const double *cr = cf + (long long)l * B;
+#pragma unroll
+            for (int t = 0; t < BMAX; ++t)
+                if (t < nb) acc[t] = fma(bl, cr[t], acc[t]);
+        }
+    }
+    const double sc = T.sign * (T.half ? sqrt(2.0 * w / kPi) : 1.0);
+    double *g = T.gs + (long long)it.col0 * T.ld_gs + i;
+#pragma unroll
+    for (int t = 0; t < BMAX; ++t)
+        if (t < nb) atomicAdd(g + (long long)t * T.ld_gs, sc * acc[t]);
 }
 
 // Spread, source-dense boxes, on the matrix cores.  A block of 4 waves
