@@ -942,7 +942,9 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 const NufhtImplHost *h = (const NufhtImplHost *)in[A.tf].plan->impl;
                 double smin = h->w[A.i0] - A.D, smax = h->w[A.i0 + A.ni - 1] - A.D;
                 int mlo = (int)std::ceil((smin - A.a2) / A.delta) - 4;
-                int mhi = (int)std::ceil((smax - A.a2) / A.delta) + T.kw + 4;
+                // (up to the last target's first tap + 16: a 16-position
+                // interpolation window may reach that far past the supports)
+                int mhi = (int)std::ceil((smax - A.a2) / A.delta) + std::max(T.kw + 4, 16);
                 if (mhi - mlo + 1 >= A.Nf) { mlo = 0; mhi = A.Nf - 1; }
                 int4 info = make_int4(A.L, mlo, mhi - mlo, 0);
                 for (long long k = 0; k < ch; ++k) P->fft_chinfo.push_back(info);
@@ -1055,7 +1057,11 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 double sf = h->w[A.i0 + s] - A.D, sl = h->w[A.i0 + s + cnt - 1] - A.D;
                 int m0f = (int)std::ceil((sf - A.a2) / A.delta);
                 int m0l = (int)std::ceil((sl - A.a2) / A.delta);
-                if (P->knobs.interp_mfma && m0l - m0f + T.kw + 2 <= 16) {
+                // window of the tile's supports (one position of slack) within
+                // 16 positions: the MFMA kernel (a 24-position form measured
+                // slower than the VALU kernel on the tiles it would add)
+                const int win = m0l - m0f + T.kw + 2;
+                if (P->knobs.interp_mfma && win <= 16) {
                     flush(s);
                     // one item per tile covers every column chunk
                     if (c0 == 0) {
