@@ -114,15 +114,14 @@ int nufht_amd_plan_stats(const NufhtPlan *plan, int batch, NufhtPlanStats *st);
  * the arithmetic's accuracy target:
  *   dense_cell    sources per grid position from which an asymptotic box
  *                 spreads on the matrix cores (else the VALU spread kernel)
- *   dense_split   sources per matrix-core spread item
+ *   dense_split   consecutive sources per matrix-core spread item
  *   spread_split  sources per VALU spread item
  *   interp_mfma   1: 16-target tiles whose kernel supports fit a 16-position
  *                 window interpolate on the matrix cores; 0: VALU only
  *   own_fft       1: two-pass FFT kernels for the lengths they cover; 0: rocFFT
  *   spread_sched  matrix-core spread schedule: 1 split (4 waves stage source
  *                 chunks into double-buffered LDS while 4 multiply), 0 unified
- *                 (4 waves stage, then multiply), -1 automatic (split for FP64
- *                 grids with at least 3 exponent groups)
+ *                 (4 waves stage, then multiply), -1 automatic (split)
  *   precision     one of NUFHT_PREC_* (as nufht_amd_plan_set_precision)
  * A plan starts from the defaults (NUFHT_AMD_DENSE_CELL and
  * NUFHT_AMD_PRECISION override two of them). */

@@ -248,7 +248,7 @@ static const int kJTabMax = 8192;
 
 // sources per spread item before a position tile is split across items
 static const int kSpreadSplit = 2048;
-static const int kDenseSplit = 4096;
+static const int kDenseSplit = 1024;
 // sources per grid cell from which a box uses the split-K spread kernel
 static const double kDenseCell = 4.0;
 // (NUFHT_AMD_DENSE_CELL overrides the threshold)
@@ -953,7 +953,6 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         P->fft.push_back(c);
     }
     P->grid_elems = off;
-    int g_dense_max = 0;   // exponent groups of the matrix-core spread's boxes
     for (auto &ap : apend) {
         int box = (int)P->abox.size();
         const AsyBoxD &A = ap.b;
@@ -964,11 +963,9 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         int sp = 256 / std::max(1, tb / 2);            // spread positions per item
         const double *rb = h->r + A.j0, *re = h->r + A.j0 + A.nj;
         // source-dense boxes (many sources per grid cell) use the split-K kernel
-        bool dense = (double)A.nj >= P->knobs.dense_cell * (2.0 * A.L + 1.0);
-        if (dense) {
-            int nsub = 16 / tb;
-            g_dense_max = std::max(g_dense_max, (A.E + 4 * nsub - 1) / (4 * nsub));
-        }
+        // (the matrix-core spread slides a 16-position window: kw <= 13 keeps
+        // the kw + 1 taps of a source inside it with room to move)
+        bool dense = T.kw <= 13 && (double)A.nj >= P->knobs.dense_cell * (2.0 * A.L + 1.0);
         // sources whose kernel support reaches grid positions [s, s + np)
         auto src_range = [&](int s, int np) {
             int lmin = INT32_MAX, lmax = INT32_MIN;
@@ -1003,26 +1000,30 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 pr.push_back({0, A.Nf});
             }
             if (dense) {
+                // the box's grid rows are zeroed, then pieces of dense_split
+                // consecutive sources each write the positions their kernels
+                // reach: plain stores where no other piece reaches (host and
+                // device first taps may differ by one: a margin of one
+                // position), atomic adds elsewhere
                 for (auto &rg : pr)
-                for (int s = rg.first; s < rg.second; s += kSP) {
-                    int np = std::min(kSP, rg.second - s);
+                    for (int s = rg.first; s < rg.second; s += 256)
+                        P->azero.push_back({box, s, std::min(256, rg.second - s), c0, tb, 0});
+                auto lf_of = [&](int j) {
+                    return (int)std::ceil((h->r[j] - A.C - A.a1) / A.h);
+                };
+                const int split = P->knobs.dense_split;
+                const int npiece = (A.nj + split - 1) / split;
+                for (int pk = 0; pk < npiece; ++pk) {
+                    int j0 = A.j0 + pk * split, j1 = std::min(A.j0 + A.nj, j0 + split);
                     SpreadRange sr;
-                    sr.piece = src_range(s, np);
-                    for (int t = 0; t < 4; ++t)
-                        sr.tile[t] = 4 * t < np ? src_range(s + 4 * t, std::min(4, np - 4 * t))
-                                                : make_int2(0, 0);
-                    int jlo = sr.piece.x, jhi = sr.piece.y;
-                    if (jhi - jlo <= P->knobs.dense_split) {
-                        P->adense.push_back({box, s, np, c0, tb, 0});
-                        P->adrange.push_back(sr);
-                    } else {
-                        P->azero.push_back({box, s, np, c0, tb, 0});
-                        for (int j = jlo; j < jhi; j += P->knobs.dense_split) {
-                            sr.piece = make_int2(j, std::min(jhi, j + P->knobs.dense_split));
-                            P->adense.push_back({box, s, np, c0, tb, 1});
-                            P->adrange.push_back(sr);
-                        }
-                    }
+                    sr.piece = make_int2(j0, j1);
+                    sr.wr = make_int2(std::max(-A.L, lf_of(j0) - 1),
+                                      std::min(A.L, lf_of(j1 - 1) + T.kw + 1));
+                    sr.ex = make_int2(pk > 0 ? lf_of(j0 - 1) + T.kw + 2 : sr.wr.x,
+                                      pk + 1 < npiece ? lf_of(j1) - 2 : sr.wr.y);
+                    sr.pad = make_int2(0, 0);
+                    P->adense.push_back({box, 0, 0, c0, tb, 0});
+                    P->adrange.push_back(sr);
                 }
             } else {
                 for (auto &rg : pr)
@@ -1079,10 +1080,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     }
     P->stats.fft_classes = (long long)P->fft.size();
     P->emax = e_max <= 6 ? 6 : e_max <= 10 ? 10 : e_max <= 14 ? 14 : 22;
-    // split schedule of the matrix-core spread where its chunks carry enough
-    // products: FP64 grids with at least 3 exponent groups
-    P->spread_split = P->knobs.spread_sched >= 0 ? P->knobs.spread_sched == 1
-                                                 : (!mixed && g_dense_max >= 3);
+    // split (producer / consumer) schedule of the matrix-core spread unless
+    // the knob asks for the unified one: it measured faster on every workload
+    // (the clustered one in mixed precision: 3.90 -> 3.53 ms per step)
+    P->spread_split = P->knobs.spread_sched != 0;
     // local-expansion launch geometry: projection items of loc_ch sources x
     // 16 columns with ceil((K+1)/16) row tiles; evaluation items of loc_tpb
     // targets x all columns, the per-target scratch within 48 KiB of LDS
@@ -1878,7 +1879,7 @@ extern "C" int nufht_multi_get_knobs(const NufhtMulti *mt, NufhtAmdKnobs *k)
 
 // Candidate values per knob, tried in this order by coordinate descent.
 static const double kTuneDense[] = {2.0, 4.0, 8.0, 1e300};
-static const int kTuneDenseSplit[] = {2048, 4096, 8192};
+static const int kTuneDenseSplit[] = {512, 1024, 2048, 4096};
 static const int kTuneSpreadSplit[] = {1024, 2048, 4096};
 
 // Times `run` (which enqueues one call on st): one untimed call (builds the

@@ -831,78 +831,97 @@ __global__ __launch_bounds__(256) void k_loc_eval(const TfDesc *__restrict__ tfs
         if (t < nb) atomicAdd(g + (long long)t * T.ld_gs, sc * acc[t]);
 }
 
-// Spread, source-dense boxes, on the matrix cores.  A block of 4 waves
-// covers kSP = 16 consecutive grid positions (wave w: positions 4w..4w+3 of
-// the item), columns [col0, col0 + tb) and a source range (the union of the
-// waves' ranges, or one piece of it when split); with nsub = 16 / tb:
+// Spread, source-dense boxes, on the matrix cores.  A block takes a range
+// of consecutive sources of a box (a piece) and columns [col0, col0 + tb);
+// with nsub = 16 / tb:
 //   G[p][e][t] = sum_q psi(l_p h - x_q) u_q^(e) c[q][t],  u^(e) = e^{iDx} r^{-1/2} (rho/r)^e
-// as matrix products over q: rows (p, e1) with e1 < 4, columns (t, e2) with
-// e2 < nsub, e = (g*4 + e1)*nsub + e2 for exponent group g; A carries
-// psi u^((4g+e1) nsub), B carries c (rho/r)^{e2}.  Re and Im are separate
-// products sharing B.  Per source, the phase, powers, kernel taps and
-// coefficients are computed once per block and shared through LDS.
+// as 16 x 16 matrix products over q, one per (exponent group eg < ceil(E /
+// nsub), real or imaginary part): rows j = (t, e2) with e2 < nsub and e =
+// eg nsub + e2, A[j][q] = c[q][t] (rho/r_q)^{e2}; columns = 16 grid
+// positions (a window that slides with the sources), B[q][p] = psi(l_p h -
+// x_q) Re or Im u_q^(eg nsub).  Each source is multiplied once, with the
+// kw + 1 of the 16 columns its kernel reaches.  The block's consumer waves
+// split the products between them and each runs every source, so that all
+// of them stay busy on every staged chunk.  Per source, the phase, powers,
+// kernel values and the A operand are computed once per block and shared
+// through LDS.
 constexpr int kSQ = 64;           // sources staged per LDS pass
-constexpr int kSP = 16;           // grid positions per block
+constexpr int kSP = 16;           // accumulator columns: a window of grid positions
 
-struct SpreadRange { int2 piece; int2 tile[4]; };
+// Sources [piece.x, piece.y) of a box; the item writes grid positions l in
+// [wr.x, wr.y], with plain stores in [ex.x, ex.y] (no other item reaches
+// them) and atomic adds elsewhere (zeroed beforehand).
+struct SpreadRange { int2 piece, wr, ex, pad; };
 
 // Row strides are padded so that both the staging writes (one source per
 // lane) and the product reads (4 sources x 4 positions or exponents per
 // instruction) spread over the LDS banks.
 template <typename R, int EMAX> struct SpreadLds {
     typedef typename Cplx<R>::T C2;
-    R psi[kSQ][kSP + 5];          // kernel value of source q at the block's positions
+    R psi[kSQ][kSP + 5];          // kernel value of source q at position l: column l mod 16
     C2 u[kSQ][EMAX + 1];          // e^{iDx} r^{-1/2} (rho/r)^e
-    R bop[kSQ][17];               // B operand: c[q][t] (rho/r)^{e2}, column j = (t, e2)
+    R bop[kSQ][17];               // A operand: c[q][t] (rho/r)^{e2}, row j = (t, e2)
+    int lf[kSQ];                  // first kernel tap of source q
 };
 
-// Products of one staged chunk for NG exponent groups.  Sources outside a
-// wave's range have zero kernel values at its positions, and the staged
-// arrays are zero past the chunk's end, so the loop needs no masks: the
-// accumulators stay in registers across steps.
-template <typename R, int EMAX, int NG>
-__device__ __forceinline__ void spread_chunk(const SpreadLds<R, EMAX> &L, int qlo, int qhi,
-                                             int prow, int E, int nsub, int ka, int e1a, int jb,
-                                             typename Mfma<R>::V (&acc_r)[NG],
-                                             typename Mfma<R>::V (&acc_i)[NG])
+// Products of one staged chunk: KC of the block's products, accumulated over
+// the chunk's sources 4 per step; uoff[k] indexes product k's part of u in a
+// source's staged row.  Rows past the chunk's end are zero, so the loop needs
+// no masks and the accumulators stay in registers.
+template <typename R, int EMAX, int KC, int KA>
+__device__ __forceinline__ void spread_chunk(const SpreadLds<R, EMAX> &L, int qhi, int lane,
+                                             const int (&uoff)[KA],
+                                             typename Mfma<R>::V (&acc)[KA])
 {
-    typedef typename Cplx<R>::T C2;
-    int eidx[NG];
-    bool eok[NG];
+    const int ka = lane >> 4, ja = lane & 15;
+    int s = 0;
+    // two steps per iteration: the second step's operands load while the
+    // first step's products run
+    for (; s + 8 <= qhi; s += 8) {
+        const int q0 = s + ka, q1 = q0 + 4;
+        const R a0 = L.bop[q0][ja], a1 = L.bop[q1][ja];
+        const R p0 = L.psi[q0][ja], p1 = L.psi[q1][ja];
+        const R *u0 = (const R *)&L.u[q0][0], *u1 = (const R *)&L.u[q1][0];
+        R b0[KC], b1[KC];
 #pragma unroll
-    for (int g = 0; g < NG; ++g) {
-        int ea = (4 * g + e1a) * nsub;
-        eok[g] = ea < E;
-        eidx[g] = ea < E ? ea : 0;
-    }
-    for (int s = qlo & ~3; s < qhi; s += 4) {
-        const int q = s + ka;
-        const R psi = L.psi[q][prow];
-        const R bval = L.bop[q][jb];
-#pragma unroll
-        for (int g = 0; g < NG; ++g) {
-            C2 u = L.u[q][eidx[g]];
-            R m = eok[g] ? psi : (R)0;
-            acc_r[g] = Mfma<R>::mma(m * u.x, bval, acc_r[g]);
-            acc_i[g] = Mfma<R>::mma(m * u.y, bval, acc_i[g]);
+        for (int k = 0; k < KC; ++k) {
+            b0[k] = p0 * u0[uoff[k]];
+            b1[k] = p1 * u1[uoff[k]];
         }
+#pragma unroll
+        for (int k = 0; k < KC; ++k) acc[k] = Mfma<R>::mma(a0, b0[k], acc[k]);
+#pragma unroll
+        for (int k = 0; k < KC; ++k) acc[k] = Mfma<R>::mma(a1, b1[k], acc[k]);
+    }
+    for (; s < qhi; s += 4) {
+        const int q = s + ka;
+        const R a = L.bop[q][ja];
+        const R ps = L.psi[q][ja];
+        const R *uq = (const R *)&L.u[q][0];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) acc[k] = Mfma<R>::mma(a, ps * uq[uoff[k]], acc[k]);
     }
 }
 
-// Spread, source-dense boxes, on the matrix cores.  A block of 4 waves
-// covers kSP = 16 consecutive grid positions (wave w: positions 4w..4w+3 of
-// the item), columns [col0, col0 + tb) and a source range (the union of the
-// waves' ranges, or one piece of it when split); with nsub = 16 / tb:
-//   G[p][e][t] = sum_q psi(l_p h - x_q) u_q^(e) c[q][t],  u^(e) = e^{iDx} r^{-1/2} (rho/r)^e
-// as matrix products over q: rows (p, e1) with e1 < 4, columns (t, e2) with
-// e2 < nsub, e = (g*4 + e1)*nsub + e2 for exponent group g; A carries
-// psi u^((4g+e1) nsub), B carries c (rho/r)^{e2}.  Re and Im are separate
-// products sharing B.  Per source, the phase, powers, kernel values and the
-// B operand are computed once per block and shared through LDS.
+// One step (sources s .. s + 3) restricted to the sources s + lo .. s + hi - 1.
+template <typename R, int EMAX, int KC, int KA>
+__device__ __forceinline__ void spread_step(const SpreadLds<R, EMAX> &L, int s, int lo, int hi,
+                                            int lane, const int (&uoff)[KA],
+                                            typename Mfma<R>::V (&acc)[KA])
+{
+    const int ka = lane >> 4, ja = lane & 15;
+    const int q = s + ka;
+    const R a = L.bop[q][ja];
+    const R ps = (ka >= lo && ka < hi) ? L.psi[q][ja] : (R)0;
+    const R *uq = (const R *)&L.u[q][0];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) acc[k] = Mfma<R>::mma(a, ps * uq[uoff[k]], acc[k]);
+}
+
 // Staging of one chunk of up to kSQ sources by the 4 producer waves
 // (pw = producer wave, ptid = its thread index in 0..255): wave 0 the
 // phases and powers, wave 1 the kernel values at the block's positions,
-// waves 2-3 the B operand.  Rows past the chunk's end are zero.
+// waves 2-3 the A operand.  Rows past the chunk's end are zero.
 template <typename R, int EMAX>
 __device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx, const AsyItem &it,
                                              int c0, int cn, int pw, int ptid,
@@ -927,27 +946,27 @@ __device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx,
             rp *= ri;
         }
     } else if (pw == 1) {
+        // taps l = lf .. lf + kw at columns l mod 16 (distinct for kw < 16);
+        // rows past the chunk's end: zero values, the last source's lf
         const int q = lane;
         const double inv_h = 1.0 / bx.h;
 #pragma unroll
         for (int pi = 0; pi < kSP; ++pi) L.psi[q][pi] = (R)0;
+        const double x = T.r[c0 + min(q, cn - 1)] - bx.C;
+        const int lf = (int)ceil((x - bx.a1) * inv_h);
+        L.lf[q] = lf;
         if (q < cn) {
-            double x = T.r[c0 + q] - bx.C;
-            int lf = (int)ceil((x - bx.a1) * inv_h);
             double f = ((double)lf * bx.h - x) * inv_h + 0.5 * kw;
             R tb_[kEsN];
             cheb_basis((R)(2.0 * f - 1.0), tb_);
             const double *tc = ft + T.ft_off + kNFT;
             for (int k = 0; k <= kw; ++k) {
                 int l = lf + k;
-                int pos = l >= 0 ? l : l + bx.Nf;
-                int pi = pos - it.start;
-                if (pi >= 0 && pi < it.count && l >= -bx.L && l <= bx.L)
-                    L.psi[q][pi] = es_tap(tc + k * kEsN, tb_);
+                if (l >= -bx.L && l <= bx.L) L.psi[q][l & 15] = es_tap(tc + k * kEsN, tb_);
             }
         }
     } else {
-        // B operand: 8 (source, column) entries per thread, loads issued 4 at a time
+        // A operand: 8 (source, row) entries per thread, loads issued 4 at a time
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             double cv[4];
@@ -977,28 +996,30 @@ __device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx,
 }
 
 // Two schedules of the same work.  SPLIT: a block of 8 waves,
-// waves 0-3 (consumers) each own 4 of the block's 16 positions and hold the
-// accumulators, waves 4-7 (producers) stage source chunks; chunks are
-// double-buffered in LDS (the consumers multiply chunk k out of buffer
-// k mod 2 while the producers stage chunk k + 1 into the other; one block
-// barrier per chunk), and the staging code's registers never coexist with
-// the accumulators (109 instead of 185 VGPRs at FP64, EMAX = 10); it pays
-// where each chunk carries enough products (FP64 grids with 3 or more
-// exponent groups: 5.98 -> 5.46 ms per headline step) and loses where
-// staging dominates (FP64 with 2 groups: 6.23 -> 6.86 ms per clustered
-// step).  Otherwise: a block of 4 waves that stage a chunk together, one
-// task each, then multiply it.
-template <typename R, int EMAX, int NG, bool SPLIT>
+// waves 0-3 (consumers) hold the accumulators, waves 4-7 (producers) stage
+// source chunks; chunks are double-buffered in LDS (the consumers multiply
+// chunk k out of buffer k mod 2 while the producers stage chunk k + 1 into
+// the other; one block barrier per chunk), and the staging code's registers
+// never coexist with the accumulators.  Otherwise: a block of 4 waves that
+// stage a chunk together, one task each, then multiply it.  KW = products
+// per consumer wave (wave w takes products [m0, m0 + mc), mc = KW or KW - 1).
+//
+// The 16 accumulator columns are a window sliding over the grid: column c
+// holds position P(c) = W + ((c - W) mod 16), the one of [W, W + 16) that is
+// c mod 16.  Sources come in ascending order (so do their first taps lf);
+// before a source's products the window start W moves up to lf + kw - 15 (if
+// it is not there already), which keeps W <= lf: the positions it passes
+// receive nothing more, and are written out and their columns cleared.
+template <typename R, int EMAX, int KW, bool SPLIT>
 __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD &bx,
-                                                 const AsyItem &it, int2 piece,
-                                                 const SpreadRange *__restrict__ rg,
-                                                 typename Cplx<R>::T *__restrict__ grid,
+                                                 const AsyItem &it, const SpreadRange &rg,
+                                                 int nprod, typename Cplx<R>::T *__restrict__ grid,
                                                  const double *__restrict__ ft,
                                                  SpreadLds<R, EMAX> *L)
 {
-    typedef typename Cplx<R>::T C2;
     typedef typename Mfma<R>::V V;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int2 piece = rg.piece;
     const int nchunk = (piece.y - piece.x + kSQ - 1) / kSQ;
     if constexpr (SPLIT) {
         const bool producer = wv >= 4;
@@ -1018,53 +1039,96 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
         }
     }
     const int tb = it.tb, nsub = 16 / tb;
-    const int E = bx.E;
-    const int ka = lane >> 4, ia = lane & 15;
-    const int pa = ia >> 2, e1a = ia & 3;
-    const int jb = lane & 15;
-    const int2 trg = rg->tile[wv];
-    const int qa = max(trg.x, piece.x), qb = min(trg.y, piece.y);
-    V acc_r[NG], acc_i[NG];
+    const int E = bx.E, kw = T.kw;
+    // this wave's products: the nprod products split as evenly as possible
+    const int base = nprod >> 2, rem = nprod & 3;
+    const int m0 = wv * base + min(wv, rem), mc = base + (wv < rem ? 1 : 0);
+    int uoff[KW];
+    V acc[KW];
 #pragma unroll
-    for (int g = 0; g < NG; ++g)
+    for (int k = 0; k < KW; ++k) {
+        const int m = min(m0 + k, nprod - 1);
+        uoff[k] = 2 * (m >> 1) * nsub + (m & 1);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) { acc_r[g][v] = (R)0; acc_i[g][v] = (R)0; }
+        for (int v = 0; v < 4; ++v) acc[k][v] = (R)0;
+    }
+    const int col = lane & 15;
+    int W = 0;
+    // positions [W, Wn) are final: write this lane's column if it holds one
+    auto flush = [&](int Wn) {
+        const int P = W + ((col - W) & 15);
+        if (P >= Wn) return;
+        if (P >= rg.wr.x && P <= rg.wr.y) {
+            const bool excl = P >= rg.ex.x && P <= rg.ex.y;
+            const int pos = P >= 0 ? P : P + bx.Nf;
+            const double dec = 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)P * bx.h * bx.a2, T.kmax));
+#pragma unroll
+            for (int k = 0; k < KW; ++k) {
+                if (k >= mc) break;
+                const int m = m0 + k, eg = m >> 1, part = m & 1;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int j = Mfma<R>::row(lane, v);
+                    const int t = it.col0 + j % tb, e = eg * nsub + j / tb;
+                    if (e >= E || t >= T.B) continue;
+                    R *d = (R *)(grid + bx.goff + ((long long)(e * T.B + t)) * bx.Nf + pos) + part;
+                    const R val = (R)((double)acc[k][v] * dec);
+                    if (excl) *d = val;
+                    else atomicAdd(d, val);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KW; ++k)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[k][v] = (R)0;
+    };
+    auto advance = [&](int Wn) {
+        if (Wn > W) {
+            flush(Wn);
+            W = Wn;
+        }
+    };
     for (int k = 0; k < nchunk; ++k) {
         const int c0 = piece.x + k * kSQ, cn = min(kSQ, piece.y - c0);
-        const int qlo = max(qa, c0) - c0, qhi = min(qb, c0 + cn) - c0;
         if constexpr (!SPLIT) {
             spread_stage<R, EMAX>(T, bx, it, c0, cn, wv, tid, ft, L[0]);
             __syncthreads();
         }
-        spread_chunk<R, EMAX, NG>(L[SPLIT ? (k & 1) : 0], qlo, qhi, 4 * wv + pa, E, nsub, ka, e1a,
-                                  jb, acc_r, acc_i);
-        __syncthreads();
-    }
-    // ---- result element (row i = (p, e1), column j = (t, e2)) of group g
-#pragma unroll
-    for (int g = 0; g < NG; ++g) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            int i = Mfma<R>::row(lane, v), j = lane & 15;
-            int p = 4 * wv + (i >> 2), e1 = i & 3, t = it.col0 + j % tb, e2 = j / tb;
-            int e = (4 * g + e1) * nsub + e2;
-            if (p >= it.count || e >= E || t >= T.B) continue;
-            int pos = it.start + p;
-            int l = pos <= bx.L ? pos : pos - bx.Nf;
-            bool valid = l >= -bx.L && l <= bx.L;
-            double dec = valid ? 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)l * bx.h * bx.a2, T.kmax)) : 0.0;
-            C2 *d = grid + bx.goff + ((long long)(e * T.B + t)) * bx.Nf + pos;
-            R vr = (R)((double)acc_r[g][v] * dec), vi = (R)((double)acc_i[g][v] * dec);
-            if (it.pad) {
-                if (valid) grid_add(d, vr, vi);
-            } else {
-                C2 w;
-                w.x = vr;
-                w.y = vi;
-                *d = w;
+        const SpreadLds<R, EMAX> &Lk = L[SPLIT ? (k & 1) : 0];
+        const int l0 = __builtin_amdgcn_readfirstlane(Lk.lf[0]);
+        const int l1 = __builtin_amdgcn_readfirstlane(Lk.lf[cn - 1]);
+        if (k == 0) W = l0 + kw - 15;
+        if (l1 - l0 <= 15 - kw) {
+            // the whole chunk fits one window
+            advance(l1 + kw - 15);
+            if (mc == KW) spread_chunk<R, EMAX, KW, KW>(Lk, cn, lane, uoff, acc);
+            else if constexpr (KW > 1) {
+                if (mc == KW - 1) spread_chunk<R, EMAX, KW - 1, KW>(Lk, cn, lane, uoff, acc);
+            }
+        } else {
+            // step by step; a step whose sources do not fit one window runs
+            // in parts, each after the window has moved for its first source
+            for (int s = 0; s < cn; s += 4) {
+                int lo = 0;
+                while (lo < 4) {
+                    const int llo = __builtin_amdgcn_readfirstlane(Lk.lf[min(s + lo, cn - 1)]);
+                    advance(llo + kw - 15);
+                    int hi = lo + 1;
+                    while (hi < 4 &&
+                           __builtin_amdgcn_readfirstlane(Lk.lf[min(s + hi, cn - 1)]) + kw <= W + 15)
+                        ++hi;
+                    if (mc == KW) spread_step<R, EMAX, KW, KW>(Lk, s, lo, hi, lane, uoff, acc);
+                    else if constexpr (KW > 1) {
+                        if (mc == KW - 1) spread_step<R, EMAX, KW - 1, KW>(Lk, s, lo, hi, lane, uoff, acc);
+                    }
+                    lo = hi;
+                }
             }
         }
+        __syncthreads();
     }
+    if (nchunk > 0) flush(W + 16);
 }
 
 // Block size and LDS bytes of the two schedules.
@@ -1076,6 +1140,20 @@ template <bool SPLIT> struct SpreadSched {
     }
 };
 
+template <typename R, int EMAX, bool SP, int KW>
+__device__ __forceinline__ void spread_mfma_dispatch(int kw, const TfDesc &T, const AsyBoxD &bx,
+                                                     const AsyItem &it, const SpreadRange &rg,
+                                                     int nprod,
+                                                     typename Cplx<R>::T *__restrict__ grid,
+                                                     const double *__restrict__ ft,
+                                                     SpreadLds<R, EMAX> *L)
+{
+    if constexpr (KW >= 1) {
+        if (kw == KW) spread_mfma_body<R, EMAX, KW, SP>(T, bx, it, rg, nprod, grid, ft, L);
+        else spread_mfma_dispatch<R, EMAX, SP, KW - 1>(kw, T, bx, it, rg, nprod, grid, ft, L);
+    }
+}
+
 template <typename R, int EMAX, bool SP>
 __global__ __launch_bounds__(SpreadSched<SP>::threads) void k_asy_spread_mfma(
     const TfDesc *__restrict__ tfs, const AsyBoxD *__restrict__ boxes,
@@ -1086,26 +1164,16 @@ __global__ __launch_bounds__(SpreadSched<SP>::threads) void k_asy_spread_mfma(
     extern __shared__ unsigned char s_spread[];
     SpreadLds<R, EMAX> *L = (SpreadLds<R, EMAX> *)s_spread;
     const AsyItem it = items[blockIdx.x];
-    const int2 piece = ranges[blockIdx.x].piece;
+    const SpreadRange rg = ranges[blockIdx.x];
     const AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
     const int nsub = 16 / it.tb;
-    const int G = (bx.E + 4 * nsub - 1) / (4 * nsub);
-    const SpreadRange *rg = ranges + blockIdx.x;
-    // the group count is uniform per block: one branch-free body per count
-    constexpr int GMAX = (EMAX + 3) / 4;
-    if (G <= 1) spread_mfma_body<R, EMAX, 1, SP>(T, bx, it, piece, rg, grid, ft, L);
-    else if (G == 2 || GMAX == 2) spread_mfma_body<R, EMAX, 2, SP>(T, bx, it, piece, rg, grid, ft, L);
-    else if constexpr (GMAX >= 3) {
-        if (G == 3 || GMAX == 3) spread_mfma_body<R, EMAX, 3, SP>(T, bx, it, piece, rg, grid, ft, L);
-        else if constexpr (GMAX >= 4) {
-            if (G == 4 || GMAX == 4) spread_mfma_body<R, EMAX, 4, SP>(T, bx, it, piece, rg, grid, ft, L);
-            else if constexpr (GMAX >= 6) {
-                if (G == 5) spread_mfma_body<R, EMAX, 5, SP>(T, bx, it, piece, rg, grid, ft, L);
-                else spread_mfma_body<R, EMAX, 6, SP>(T, bx, it, piece, rg, grid, ft, L);
-            }
-        }
-    }
+    // products: 2 per exponent group; the per-wave count is uniform per
+    // block, one branch-free body per count
+    const int nprod = 2 * ((bx.E + nsub - 1) / nsub);
+    const int kw = (nprod + 3) >> 2;
+    constexpr int KWMAX = (2 * EMAX + 3) / 4;
+    spread_mfma_dispatch<R, EMAX, SP, KWMAX>(kw, T, bx, it, rg, nprod, grid, ft, L);
 }
 
 // Interpolation, target-dense tiles: 16 consecutive targets whose kernel
