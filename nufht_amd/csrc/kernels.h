@@ -918,20 +918,42 @@ __device__ __forceinline__ void spread_step(const SpreadLds<R, EMAX> &L, int s, 
     for (int k = 0; k < KC; ++k) acc[k] = Mfma<R>::mma(a, ps * uq[uoff[k]], acc[k]);
 }
 
+// Global loads of one producer thread for one chunk (its source's r; for
+// waves 2-3 also 8 coefficients), issued a chunk ahead of their use.
+template <bool CV> struct SpreadIn { double r; double cv[8]; };
+template <> struct SpreadIn<false> { double r; };
+
+template <bool CV>
+__device__ __forceinline__ void spread_load(const TfDesc &T, const AsyItem &it, int c0, int cn,
+                                            int ptid, SpreadIn<CV> &in)
+{
+    const int q = ptid & 63;
+    in.r = T.r[c0 + min(q, cn - 1)];
+    if constexpr (CV) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int j = (ptid - 128 + 128 * m) >> 6, tc = it.col0 + j % it.tb;
+            in.cv[m] = (q < cn && tc < T.B) ? T.cs[(long long)tc * T.ld_cs + c0 + q] : 0.0;
+        }
+    }
+}
+
 // Staging of one chunk of up to kSQ sources by the 4 producer waves
-// (pw = producer wave, ptid = its thread index in 0..255): wave 0 the
-// phases and powers, wave 1 the kernel values at the block's positions,
-// waves 2-3 the A operand.  Rows past the chunk's end are zero.
-template <typename R, int EMAX>
+// (pw = producer wave, ptid = its thread index in 0..255) from their loads:
+// wave 0 the phases and powers, wave 1 the kernel values at the positions
+// the sources reach, waves 2-3 the A operand.  Rows past the chunk's end
+// are zero.
+template <typename R, int EMAX, bool CV>
 __device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx, const AsyItem &it,
                                              int c0, int cn, int pw, int ptid,
-                                             const double *__restrict__ ft, SpreadLds<R, EMAX> &L)
+                                             const SpreadIn<CV> &in, const double *__restrict__ ft,
+                                             SpreadLds<R, EMAX> &L)
 {
     typedef typename Cplx<R>::T C2;
     const int lane = ptid & 63, tb = it.tb, E = bx.E, kw = T.kw;
     if (pw == 0) {
         const int q = lane;
-        double rj = q < cn ? T.r[c0 + q] : bx.C;
+        double rj = q < cn ? in.r : bx.C;
         double x = rj - bx.C;
         double sn, cs;
         sincos_cw(bx.D * x, &sn, &cs);
@@ -952,7 +974,7 @@ __device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx,
         const double inv_h = 1.0 / bx.h;
 #pragma unroll
         for (int pi = 0; pi < kSP; ++pi) L.psi[q][pi] = (R)0;
-        const double x = T.r[c0 + min(q, cn - 1)] - bx.C;
+        const double x = in.r - bx.C;
         const int lf = (int)ceil((x - bx.a1) * inv_h);
         L.lf[q] = lf;
         if (q < cn) {
@@ -965,33 +987,55 @@ __device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx,
                 if (l >= -bx.L && l <= bx.L) L.psi[q][l & 15] = es_tap(tc + k * kEsN, tb_);
             }
         }
-    } else {
-        // A operand: 8 (source, row) entries per thread, loads issued 4 at a time
+    } else if constexpr (CV) {
+        // A operand: 8 (source, row) entries per thread, one source
+        const int q = lane;
+        const double ri = q < cn ? bx.rho / in.r : 0.0;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            double cv[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                int idx = ptid - 128 + 128 * (4 * h + u);
-                int q = idx & (kSQ - 1), j = idx >> 6;
-                int tt = j % tb;
-                int tc = it.col0 + tt;
-                cv[u] = (q < cn && tc < T.B) ? T.cs[(long long)tc * T.ld_cs + c0 + q] : 0.0;
+        for (int m = 0; m < 8; ++m) {
+            const int j = (ptid - 128 + 128 * m) >> 6, e2 = j / tb;
+            double c = in.cv[m];
+            if (e2 > 0) {
+                double rp = ri;
+                for (int e = 1; e < e2; ++e) rp *= ri;
+                c *= rp;
             }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                int idx = ptid - 128 + 128 * (4 * h + u);
-                int q = idx & (kSQ - 1), j = idx >> 6;
-                int e2 = j / tb;
-                double c = cv[u];
-                if (e2 > 0 && q < cn) {
-                    double ri = bx.rho / T.r[c0 + q], rp = ri;
-                    for (int e = 1; e < e2; ++e) rp *= ri;
-                    c *= rp;
-                }
-                L.bop[q][j] = (R)c;
-            }
+            L.bop[q][j] = (R)c;
         }
+    }
+}
+
+// The producer side of the split schedule: stage chunk 0, then chunk k + 1
+// while the consumers multiply chunk k, one block barrier per chunk (as many
+// as the consumers pass).  The loads of chunk k + 1 are in flight while
+// chunk k is staged and the barrier waits.
+template <typename R, int EMAX, bool CV>
+__device__ __forceinline__ void spread_produce(const TfDesc &T, const AsyBoxD &bx,
+                                               const AsyItem &it, int2 piece, int nchunk, int pw,
+                                               int ptid, const double *__restrict__ ft,
+                                               SpreadLds<R, EMAX> *L)
+{
+    SpreadIn<CV> in;
+    if (nchunk > 0) {
+        spread_load(T, it, piece.x, min(kSQ, piece.y - piece.x), ptid, in);
+        spread_stage<R, EMAX>(T, bx, it, piece.x, min(kSQ, piece.y - piece.x), pw, ptid, in, ft,
+                              L[0]);
+        if (nchunk > 1) {
+            const int c1 = piece.x + kSQ;
+            spread_load(T, it, c1, min(kSQ, piece.y - c1), ptid, in);
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < nchunk; ++k) {
+        if (k + 1 < nchunk) {
+            SpreadIn<CV> nx = in;
+            const int c1 = piece.x + (k + 1) * kSQ, c2 = c1 + kSQ;
+            if (k + 2 < nchunk) spread_load(T, it, c2, min(kSQ, piece.y - c2), ptid, nx);
+            spread_stage<R, EMAX>(T, bx, it, c1, min(kSQ, piece.y - c1), pw, ptid, in, ft,
+                                  L[(k + 1) & 1]);
+            in = nx;
+        }
+        __syncthreads();
     }
 }
 
@@ -1022,21 +1066,13 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
     const int2 piece = rg.piece;
     const int nchunk = (piece.y - piece.x + kSQ - 1) / kSQ;
     if constexpr (SPLIT) {
-        const bool producer = wv >= 4;
-        if (producer && nchunk > 0)
-            spread_stage<R, EMAX>(T, bx, it, piece.x, min(kSQ, piece.y - piece.x), wv - 4,
-                                  tid - 256, ft, L[0]);
-        __syncthreads();
-        if (producer) {
-            for (int k = 0; k < nchunk; ++k) {
-                const int c1 = piece.x + (k + 1) * kSQ;
-                if (k + 1 < nchunk)
-                    spread_stage<R, EMAX>(T, bx, it, c1, min(kSQ, piece.y - c1), wv - 4, tid - 256,
-                                          ft, L[(k + 1) & 1]);
-                __syncthreads();
-            }
+        if (wv >= 4) {
+            // waves 6-7 hold the coefficients they load, waves 4-5 only r
+            if (wv >= 6) spread_produce<R, EMAX, true>(T, bx, it, piece, nchunk, wv - 4, tid - 256, ft, L);
+            else spread_produce<R, EMAX, false>(T, bx, it, piece, nchunk, wv - 4, tid - 256, ft, L);
             return;
         }
+        __syncthreads();
     }
     const int tb = it.tb, nsub = 16 / tb;
     const int E = bx.E, kw = T.kw;
@@ -1053,12 +1089,28 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
         for (int v = 0; v < 4; ++v) acc[k][v] = (R)0;
     }
     const int col = lane & 15;
+    // this lane's result rows j = (tt, e2), packed e2 << 8 | tt (-1: no column)
+    int rowpk[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int j = Mfma<R>::row(lane, v), tt = j % tb;
+        rowpk[v] = it.col0 + tt < T.B ? ((j / tb) << 8) | tt : -1;
+    }
     int W = 0;
     // positions [W, Wn) are final: write this lane's column if it holds one
     auto flush = [&](int Wn) {
         const int P = W + ((col - W) & 15);
         if (P >= Wn) return;
         if (P >= rg.wr.x && P <= rg.wr.y) {
+            // (the opaque copies keep the row and address arithmetic here,
+            // where it runs once per window move, instead of held in
+            // registers across the whole loop)
+            int pk[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                pk[v] = rowpk[v];
+                asm volatile("" : "+v"(pk[v]));
+            }
             const bool excl = P >= rg.ex.x && P <= rg.ex.y;
             const int pos = P >= 0 ? P : P + bx.Nf;
             const double dec = 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)P * bx.h * bx.a2, T.kmax));
@@ -1068,9 +1120,9 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
                 const int m = m0 + k, eg = m >> 1, part = m & 1;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int j = Mfma<R>::row(lane, v);
-                    const int t = it.col0 + j % tb, e = eg * nsub + j / tb;
-                    if (e >= E || t >= T.B) continue;
+                    const int e = eg * nsub + (pk[v] >> 8);
+                    if (pk[v] < 0 || e >= E) continue;
+                    const int t = it.col0 + (pk[v] & 255);
                     R *d = (R *)(grid + bx.goff + ((long long)(e * T.B + t)) * bx.Nf + pos) + part;
                     const R val = (R)((double)acc[k][v] * dec);
                     if (excl) *d = val;
@@ -1092,7 +1144,15 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
     for (int k = 0; k < nchunk; ++k) {
         const int c0 = piece.x + k * kSQ, cn = min(kSQ, piece.y - c0);
         if constexpr (!SPLIT) {
-            spread_stage<R, EMAX>(T, bx, it, c0, cn, wv, tid, ft, L[0]);
+            if (wv >= 2) {
+                SpreadIn<true> in;
+                spread_load(T, it, c0, cn, tid, in);
+                spread_stage<R, EMAX>(T, bx, it, c0, cn, wv, tid, in, ft, L[0]);
+            } else {
+                SpreadIn<false> in;
+                spread_load(T, it, c0, cn, tid, in);
+                spread_stage<R, EMAX>(T, bx, it, c0, cn, wv, tid, in, ft, L[0]);
+            }
             __syncthreads();
         }
         const SpreadLds<R, EMAX> &Lk = L[SPLIT ? (k & 1) : 0];
@@ -1154,8 +1214,10 @@ __device__ __forceinline__ void spread_mfma_dispatch(int kw, const TfDesc &T, co
     }
 }
 
+// (split schedule, EMAX <= 10: at most 128 VGPRs, so that two blocks share a
+// CU; the few values that then leave registers are reloaded once per chunk)
 template <typename R, int EMAX, bool SP>
-__global__ __launch_bounds__(SpreadSched<SP>::threads) void k_asy_spread_mfma(
+__global__ __launch_bounds__(SpreadSched<SP>::threads, (SP && EMAX <= 10) ? 4 : 1) void k_asy_spread_mfma(
     const TfDesc *__restrict__ tfs, const AsyBoxD *__restrict__ boxes,
     const AsyItem *__restrict__ items, const SpreadRange *__restrict__ ranges,
     typename Cplx<R>::T *__restrict__ grid, const double *__restrict__ ft)
