@@ -114,7 +114,8 @@ int nufht_amd_plan_stats(const NufhtPlan *plan, int batch, NufhtPlanStats *st);
  * the arithmetic's accuracy target:
  *   dense_cell    sources per grid position from which an asymptotic box
  *                 spreads on the matrix cores (else the VALU spread kernel)
- *   dense_split   consecutive sources per matrix-core spread item
+ *   dense_split   consecutive sources per matrix-core spread item (0: 512 for
+ *                 kernel widths <= 8, else 1024)
  *   spread_split  sources per VALU spread item
  *   interp_mfma   1: 16-target tiles whose kernel supports fit a 16-position
  *                 window interpolate on the matrix cores; 0: VALU only

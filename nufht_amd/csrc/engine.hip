@@ -248,7 +248,7 @@ static const int kJTabMax = 8192;
 
 // sources per spread item before a position tile is split across items
 static const int kSpreadSplit = 2048;
-static const int kDenseSplit = 1024;
+static const int kDenseSplit = 0;      // 0: from the plan's size
 // sources per grid cell from which a box uses the split-K spread kernel
 static const double kDenseCell = 4.0;
 // (NUFHT_AMD_DENSE_CELL overrides the threshold)
@@ -953,6 +953,22 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         P->fft.push_back(c);
     }
     P->grid_elems = off;
+    // sources per matrix-core spread item: the knob, or (0) 512 where every
+    // dense box has kw <= 8 (tol >= 1e-6; the clustered workload's spread
+    // 3.08 -> 2.73 ms per step in mixed precision, 4.06 -> 3.87 in FP64,
+    // against 1024) and 1024 otherwise (the headline: 3.02, against 3.21 at
+    // 512); the autotuner tries others
+    int dense_split = P->knobs.dense_split;
+    if (dense_split == 0) {
+        int kwd = 0;
+        for (auto &ap : apend) {
+            const AsyBoxD &A = ap.b;
+            const TfDesc &T = P->tfs[A.tf];
+            if (T.kw <= 13 && (double)A.nj >= P->knobs.dense_cell * (2.0 * A.L + 1.0))
+                kwd = std::max(kwd, T.kw);
+        }
+        dense_split = kwd <= 8 ? 512 : 1024;
+    }
     for (auto &ap : apend) {
         int box = (int)P->abox.size();
         const AsyBoxD &A = ap.b;
@@ -1011,7 +1027,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 auto lf_of = [&](int j) {
                     return (int)std::ceil((h->r[j] - A.C - A.a1) / A.h);
                 };
-                const int split = P->knobs.dense_split;
+                const int split = dense_split;
                 const int npiece = (A.nj + split - 1) / split;
                 for (int pk = 0; pk < npiece; ++pk) {
                     int j0 = A.j0 + pk * split, j1 = std::min(A.j0 + A.nj, j0 + split);
@@ -1844,7 +1860,8 @@ extern "C" int nufht_amd_get_knobs(const NufhtPlan *plan, NufhtAmdKnobs *k)
 
 static int knobs_check(const NufhtAmdKnobs *k)
 {
-    if (!(k->dense_cell > 0.0) || k->dense_split < kSQ || k->spread_split < 64 ||
+    if (!(k->dense_cell > 0.0) || (k->dense_split != 0 && k->dense_split < kSQ) ||
+        k->spread_split < 64 ||
         (k->interp_mfma != 0 && k->interp_mfma != 1) || (k->own_fft != 0 && k->own_fft != 1) ||
         k->spread_sched < -1 || k->spread_sched > 1 ||
         (k->precision != NUFHT_PREC_FP64 && k->precision != NUFHT_PREC_AUTO &&

@@ -34,6 +34,8 @@ ALTERNATIVES = [
     {"own_fft": 0},                 # rocFFT for every length
     {"dense_split": 2048, "spread_split": 1024},
     {"dense_split": 8192, "spread_split": 4096},
+    {"dense_split": 64},            # matrix-core spread: the smallest pieces (most boundaries)
+    {"dense_split": 0},             # matrix-core spread: pieces sized from the kernel width
     {"spread_sched": 0},            # matrix-core spread: unified schedule
     {"spread_sched": 1},            # matrix-core spread: split schedule
 ]
