@@ -274,30 +274,33 @@ struct Knobs {
     bool operator!=(const Knobs &o) const { return !(*this == o); }
 };
 
-// FFT lengths restricted to 2^a and 3*2^a (prebuilt rocFFT kernels)
+// FFT lengths: 2^a and 3 2^a (prebuilt rocFFT kernels), and above 4096 (the
+// two-pass FFT kernels' range) 9 2^a
 static int fft_length(long long need)
 {
     long long best = 1LL << 40;
     for (long long p = 8; p < (1LL << 34); p <<= 1) {
         if (p >= need) { best = std::min(best, p); break; }
         if (3 * p >= need) best = std::min(best, 3 * p);
+        if (9 * p >= need && 9 * p > 4096) best = std::min(best, 9 * p);
     }
     return (int)best;
 }
 
-// Two-pass factorisation Nf = N1 N2 for the FFT kernels (Nf = 2^a or 3 2^a
-// above 4096; the factor 3 goes into N1), sequences per block and radices.
+// Two-pass factorisation Nf = N1 N2 for the FFT kernels (Nf = 2^a, 3 2^a or
+// 9 2^a above 4096; the factor 3 or 9 goes into N1), sequences per block and
+// radices.
 static void fft_radices(int N, FftPass &p)
 {
     p.nstage = 0;
-    int three = N % 3 == 0;
-    if (three) N /= 3;
+    const int f3 = N % 9 == 0 ? 9 : N % 3 == 0 ? 3 : 1;
+    N /= f3;
     int k = 0;
     while ((1 << k) < N) ++k;
     for (int q = 0; q < k / 3; ++q) p.radix[p.nstage++] = 8;
     if (k % 3 == 2) p.radix[p.nstage++] = 4;
     if (k % 3 == 1) p.radix[p.nstage++] = 2;
-    if (three) p.radix[p.nstage++] = 3;
+    if (f3 > 1) p.radix[p.nstage++] = f3;
 }
 static int fft_pick_m(int N, int other, int maxe)
 {
@@ -309,7 +312,8 @@ static int fft_pick_m(int N, int other, int maxe)
 static bool fft_factor(int nf, int maxe, FftPass &p1, FftPass &p2)
 {
     if (nf <= 4096) return false;
-    int b3 = nf % 3 == 0, q = b3 ? nf / 3 : nf, a = 0;
+    const int f3 = nf % 9 == 0 ? 9 : nf % 3 == 0 ? 3 : 1;
+    int q = nf / f3, a = 0;
     while ((1 << a) < q) ++a;
     if ((1 << a) != q) return false;
     // N2 = 2^j; the factorisation with the widest contiguous runs (min M)

@@ -20,7 +20,8 @@ namespace nufht_dev {
 
 constexpr int kFftThreads = 512;
 // M * N per block: 64 KiB of data, so that two blocks share a CU and one
-// block's loads overlap the other's LDS passes
+// block's loads overlap the other's LDS passes (the kernels are bounded to
+// 128 VGPRs, 4 waves per SIMD, for the same reason)
 constexpr int kFftTileBytes = 65536;
 template <typename R> constexpr int fft_max_elems() { return kFftTileBytes / (2 * (int)sizeof(R)); }
 constexpr int kFftMaxStages = 8;
@@ -31,7 +32,7 @@ struct FftPass {
     int N1, N2;               // the class's factorisation, Nf = N1 * N2
     int tiles;                // blocks per channel (columns or rows / M)
     int nstage;
-    int radix[kFftMaxStages]; // powers of 2 first, then at most one 3
+    int radix[kFftMaxStages]; // powers of 2 first, then at most one 3 or 9
     int w_off;                // W_N[m] = e^{2 pi i m / N}, m < N
     int hi_off, lo_off;       // pass 1: w^a = hi[a >> 8] lo[a & 255], a < Nf
     int chinfo_off;           // per channel {L, mlo, span}: pass 1 reads only grid positions
@@ -94,6 +95,42 @@ template <typename R> struct Dft<R, 4> {
         x[3] = CMul<R>::sub(b, d);
     }
 };
+// 9 = 3 x 3: x[3 n1 + n2] -> 3-point DFTs over n1, twiddles e^{2 pi i n2 k1 / 9},
+// 3-point DFTs over n2 into X[k1 + 3 k2]
+template <typename R> struct Dft<R, 9> {
+    typedef typename Cplx<R>::T C2;
+    static __device__ __forceinline__ void run(C2 *x)
+    {
+        C2 y[3][3];
+#pragma unroll
+        for (int n2 = 0; n2 < 3; ++n2) {
+            C2 t[3] = {x[n2], x[3 + n2], x[6 + n2]};
+            Dft<R, 3>::run(t);
+            y[n2][0] = t[0];
+            y[n2][1] = t[1];
+            y[n2][2] = t[2];
+        }
+        C2 w1, w2, w4;                            // e^{2 pi i m / 9}, m = 1, 2, 4
+        w1.x = (R)0.76604444311897803520;         // cos(2 pi / 9)
+        w1.y = (R)0.64278760968653932632;         // sin(2 pi / 9)
+        w2.x = (R)0.17364817766693034885;         // cos(4 pi / 9)
+        w2.y = (R)0.98480775301220805936;         // sin(4 pi / 9)
+        w4.x = (R)-0.93969262078590838405;        // cos(8 pi / 9)
+        w4.y = (R)0.34202014332566873304;         // sin(8 pi / 9)
+        y[1][1] = CMul<R>::mul(y[1][1], w1);
+        y[1][2] = CMul<R>::mul(y[1][2], w2);
+        y[2][1] = CMul<R>::mul(y[2][1], w2);
+        y[2][2] = CMul<R>::mul(y[2][2], w4);
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) {
+            C2 t[3] = {y[0][k1], y[1][k1], y[2][k1]};
+            Dft<R, 3>::run(t);
+            x[k1] = t[0];
+            x[k1 + 3] = t[1];
+            x[k1 + 6] = t[2];
+        }
+    }
+};
 template <typename R> struct Dft<R, 8> {
     typedef typename Cplx<R>::T C2;
     static __device__ __forceinline__ void run(C2 *x)
@@ -127,7 +164,8 @@ __host__ __device__ inline size_t fft_lds_elems(int N, int M) { return (size_t)N
 
 // One Stockham stage of radix RAD over M sequences of length N held in LDS
 // at fft_lds_idx(n, c); Ns = product of the earlier radices.
-// Ns is a power of 2 (lgNs its log): the radix-3 stage, if any, comes last.
+// Ns is a power of 2 (lgNs its log): the radix-3 or radix-9 stage, if any,
+// comes last.
 template <typename R, int RAD>
 __device__ __forceinline__ void fft_stage(typename Cplx<R>::T *s, int N, int M, int lgM, int lgNs,
                                           const typename Cplx<R>::T *__restrict__ W)
@@ -176,6 +214,7 @@ __device__ __forceinline__ void fft_lds(typename Cplx<R>::T *s, const FftPass &P
         if (rad == 8) fft_stage<R, 8>(s, P.N, P.M, P.lgM, lgNs, W);
         else if (rad == 4) fft_stage<R, 4>(s, P.N, P.M, P.lgM, lgNs, W);
         else if (rad == 3) fft_stage<R, 3>(s, P.N, P.M, P.lgM, lgNs, W);
+        else if (rad == 9) fft_stage<R, 9>(s, P.N, P.M, P.lgM, lgNs, W);
         else fft_stage<R, 2>(s, P.N, P.M, P.lgM, lgNs, W);
         lgNs += rad == 8 ? 3 : rad == 4 ? 2 : 1;
     }
@@ -183,7 +222,7 @@ __device__ __forceinline__ void fft_lds(typename Cplx<R>::T *s, const FftPass &P
 
 // Pass 1: column FFTs of length N1 (stride N2) in place, times w^{k1 n2}.
 template <typename R>
-__global__ __launch_bounds__(kFftThreads, 2) void k_fft_cols(typename Cplx<R>::T *__restrict__ g,
+__global__ __launch_bounds__(kFftThreads, 4) void k_fft_cols(typename Cplx<R>::T *__restrict__ g,
                                                           FftPass P,
                                                           const typename Cplx<R>::T *__restrict__ tw,
                                                           const int4 *__restrict__ chinfo)
@@ -226,7 +265,7 @@ __global__ __launch_bounds__(kFftThreads, 2) void k_fft_cols(typename Cplx<R>::T
 
 // Pass 2: row FFTs of length N2, written to natural order in out.
 template <typename R>
-__global__ __launch_bounds__(kFftThreads, 2) void k_fft_rows(const typename Cplx<R>::T *__restrict__ g,
+__global__ __launch_bounds__(kFftThreads, 4) void k_fft_rows(const typename Cplx<R>::T *__restrict__ g,
                                                           typename Cplx<R>::T *__restrict__ out,
                                                           FftPass P,
                                                           const typename Cplx<R>::T *__restrict__ tw,
