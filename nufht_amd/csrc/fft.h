@@ -220,6 +220,18 @@ __device__ __forceinline__ void fft_lds(typename Cplx<R>::T *s, const FftPass &P
     }
 }
 
+// Blocks go to the 8 XCDs (each with its own L2) round robin in launch
+// order; a tile's runs of M elements fill half of a 128-byte line when M
+// elements are 64 bytes, the other half belonging to the next tile.  Within
+// each group of 16 blocks, blocks b and b + 8 (same XCD) take adjacent tiles,
+// so that a line's two halves meet in one L2.
+__device__ __forceinline__ int fft_block_tile()
+{
+    const int b = blockIdx.x;
+    if (b >= (int)(gridDim.x & ~15u)) return b;
+    return (b & ~15) + 2 * (b & 7) + ((b >> 3) & 1);
+}
+
 // Pass 1: column FFTs of length N1 (stride N2) in place, times w^{k1 n2}.
 template <typename R>
 __global__ __launch_bounds__(kFftThreads, 4) void k_fft_cols(typename Cplx<R>::T *__restrict__ g,
@@ -230,7 +242,8 @@ __global__ __launch_bounds__(kFftThreads, 4) void k_fft_cols(typename Cplx<R>::T
     typedef typename Cplx<R>::T C2;
     extern __shared__ unsigned char s_raw[];
     C2 *s = (C2 *)s_raw;
-    const int ch = blockIdx.x / P.tiles, t = blockIdx.x % P.tiles;
+    const int bt = fft_block_tile();
+    const int ch = bt / P.tiles, t = bt % P.tiles;
     const int N = P.N, M = P.M;
     const int L = chinfo[P.chinfo_off + ch].x;
     const int nf = (int)P.nf;
@@ -274,7 +287,8 @@ __global__ __launch_bounds__(kFftThreads, 4) void k_fft_rows(const typename Cplx
     typedef typename Cplx<R>::T C2;
     extern __shared__ unsigned char s_raw[];
     C2 *s = (C2 *)s_raw;
-    const int ch = blockIdx.x / P.tiles, t = blockIdx.x % P.tiles;
+    const int bt = fft_block_tile();
+    const int ch = bt / P.tiles, t = bt % P.tiles;
     const int N = P.N, M = P.M;
     const C2 *src = g + (long long)ch * P.nf + (long long)t * M * P.N2;
     constexpr int EPT = fft_max_elems<R>() / kFftThreads, HALF = EPT / 2;
