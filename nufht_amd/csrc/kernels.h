@@ -317,6 +317,17 @@ template <typename R> struct Cplx;
 template <> struct Cplx<double> { typedef double2 T; };
 template <> struct Cplx<float> { typedef float2 T; };
 
+// Blocks go to the 8 XCDs (each with its own L2) round robin in launch
+// order.  Item index of this block such that within each group of 8 K blocks
+// every XCD takes K consecutive items (neighbouring items read neighbouring
+// grid windows); blocks of the last, partial group keep their own index.
+template <int K> __device__ __forceinline__ int xcd_item()
+{
+    const int b = blockIdx.x;
+    if (b >= (int)(gridDim.x - gridDim.x % (8 * K))) return b;
+    return b - b % (8 * K) + K * (b & 7) + (b >> 3) % K;
+}
+
 __device__ __forceinline__ double es_eval(double u, double beta) { return es_kernel(u, beta); }
 __device__ __forceinline__ float es_eval(float u, float beta) { return es_kernel_f(u, beta); }
 
@@ -1265,7 +1276,7 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
     __shared__ double s_P[16][EMAX];
     __shared__ double s_out[16][17];   // result tile, transposed for the atomic adds
     __shared__ int s_m0[16];
-    AsyItem it = items[blockIdx.x];
+    AsyItem it = items[xcd_item<4>()];
     AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
     const int lane = threadIdx.x;
@@ -1431,7 +1442,7 @@ __global__ __launch_bounds__(256) void k_asy_interp(const TfDesc *__restrict__ t
     // dynamic LDS, sized per launch for the widest target tile (tmax):
     // phi[tmax][16] | fr, fi, b0, rw [tmax] | m0[tmax] | grid window
     extern __shared__ double s_dyn[];
-    AsyItem it = items[blockIdx.x];
+    AsyItem it = items[xcd_item<4>()];
     AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
     int tb = it.tb, ti_n = 256 / tb;
