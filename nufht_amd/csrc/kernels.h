@@ -1236,8 +1236,9 @@ __global__ __launch_bounds__(SpreadSched<SP>::threads, (SP && EMAX <= 10) ? 4 : 
     // one or two SpreadLds buffers, dynamic (above 64 KiB for the widest EMAX)
     extern __shared__ unsigned char s_spread[];
     SpreadLds<R, EMAX> *L = (SpreadLds<R, EMAX> *)s_spread;
-    const AsyItem it = items[blockIdx.x];
-    const SpreadRange rg = ranges[blockIdx.x];
+    const int bi = xcd_item<4>();
+    const AsyItem it = items[bi];
+    const SpreadRange rg = ranges[bi];
     const AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
     const int nsub = 16 / it.tb;
@@ -1276,7 +1277,7 @@ __global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict
     __shared__ double s_P[16][EMAX];
     __shared__ double s_out[16][17];   // result tile, transposed for the atomic adds
     __shared__ int s_m0[16];
-    AsyItem it = items[xcd_item<4>()];
+    AsyItem it = items[xcd_item<16>()];
     AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
     const int lane = threadIdx.x;
