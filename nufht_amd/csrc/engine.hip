@@ -1197,8 +1197,18 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         fft_table(tw, c.nf, 256, &c.p1.lo_off);
         c.p2.hi_off = c.p1.hi_off;
         c.p2.lo_off = c.p1.lo_off;
-        c.lds1 = fft_lds_elems(c.p1.N, c.p1.M) * esz;
-        c.lds2 = fft_lds_elems(c.p2.N, c.p2.M) * esz;
+        // LDS layout of each pass: padded rows unless that keeps a second
+        // block off the CU (above 80 KiB), then the compact layout
+        for (FftPass *fp : {&c.p1, &c.p2}) {
+            fp->lds_stride = fp->M + 1;
+            fp->lds_shift = 3;
+            if (fft_lds_elems(fp->N, fp->lds_stride, fp->lds_shift, fp->M) * esz > 80 * 1024) {
+                fp->lds_stride = fp->M;
+                fp->lds_shift = 2;
+            }
+        }
+        c.lds1 = fft_lds_elems(c.p1.N, c.p1.lds_stride, c.p1.lds_shift, c.p1.M) * esz;
+        c.lds2 = fft_lds_elems(c.p2.N, c.p2.lds_stride, c.p2.lds_shift, c.p2.M) * esz;
     }
     if (!P->fft_chinfo.empty()) RC_TRY(upload(&P->d_fft_chinfo, P->fft_chinfo));
     if (!tw.empty()) {

@@ -35,6 +35,7 @@ struct FftPass {
     int radix[kFftMaxStages]; // powers of 2 first, then at most one 3 or 9
     int w_off;                // W_N[m] = e^{2 pi i m / N}, m < N
     int hi_off, lo_off;       // pass 1: w^a = hi[a >> 8] lo[a & 255], a < Nf
+    int lds_stride, lds_shift; // LDS layout: fft_lds_idx
     int chinfo_off;           // per channel {L, mlo, span}: pass 1 reads only grid positions
                               // l in [-L, L] (the rest are zeros not stored), pass 2
                               // writes only frequencies mlo .. mlo + span (mod Nf)
@@ -156,11 +157,19 @@ template <typename R> struct Dft<R, 8> {
     }
 };
 
-// LDS position of element n of sequence c: rows of M + 1 and one more pad
-// element every 8 rows, so that consecutive n (the row pass's loads) and the
-// stride-R writes of a first stage both spread over the banks.
-__device__ __forceinline__ int fft_lds_idx(int n, int c, int M) { return n * (M + 1) + c + (n >> 3); }
-__host__ __device__ inline size_t fft_lds_elems(int N, int M) { return (size_t)N * (M + 1) + (N >> 3) + M; }
+// LDS position of element n of sequence c: rows of stride = M + 1 and one
+// more pad element every 2^shift = 8 rows, so that consecutive n (the row
+// pass's loads) and the stride-R writes of a first stage both spread over the
+// banks; where that padding would keep a second block off the CU (more than
+// 80 KiB), rows of stride = M and one pad element every 2^shift = 4 rows.
+__device__ __forceinline__ int fft_lds_idx(int n, int c, int stride, int shift)
+{
+    return n * stride + c + (n >> shift);
+}
+__host__ __device__ inline size_t fft_lds_elems(int N, int stride, int shift, int M)
+{
+    return (size_t)N * stride + (N >> shift) + M;
+}
 
 // One Stockham stage of radix RAD over M sequences of length N held in LDS
 // at fft_lds_idx(n, c); Ns = product of the earlier radices.
@@ -168,7 +177,7 @@ __host__ __device__ inline size_t fft_lds_elems(int N, int M) { return (size_t)N
 // comes last.
 template <typename R, int RAD>
 __device__ __forceinline__ void fft_stage(typename Cplx<R>::T *s, int N, int M, int lgM, int lgNs,
-                                          const typename Cplx<R>::T *__restrict__ W)
+                                          int ls, int lh, const typename Cplx<R>::T *__restrict__ W)
 {
     typedef typename Cplx<R>::T C2;
     constexpr int MAXB = (fft_max_elems<R>() / RAD + kFftThreads - 1) / kFftThreads;
@@ -182,7 +191,7 @@ __device__ __forceinline__ void fft_stage(typename Cplx<R>::T *s, int N, int M, 
         if (b < nb) {
             int c = b & (M - 1), j = b >> lgM, k = j & (Ns - 1);
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) x[q][r] = s[fft_lds_idx(j + r * span, c, M)];
+            for (int r = 0; r < RAD; ++r) x[q][r] = s[fft_lds_idx(j + r * span, c, ls, lh)];
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < RAD; ++r) x[q][r] = CMul<R>::mul(x[q][r], W[r * k * wstep]);
@@ -198,7 +207,7 @@ __device__ __forceinline__ void fft_stage(typename Cplx<R>::T *s, int N, int M, 
             int c = b & (M - 1), j = b >> lgM, k = j & (Ns - 1);
             int base = (j >> lgNs) * Ns * RAD + k;
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) s[fft_lds_idx(base + r * Ns, c, M)] = x[q][r];
+            for (int r = 0; r < RAD; ++r) s[fft_lds_idx(base + r * Ns, c, ls, lh)] = x[q][r];
         }
     }
     __syncthreads();
@@ -211,11 +220,11 @@ __device__ __forceinline__ void fft_lds(typename Cplx<R>::T *s, const FftPass &P
     int lgNs = 0;
     for (int st = 0; st < P.nstage; ++st) {
         int rad = P.radix[st];
-        if (rad == 8) fft_stage<R, 8>(s, P.N, P.M, P.lgM, lgNs, W);
-        else if (rad == 4) fft_stage<R, 4>(s, P.N, P.M, P.lgM, lgNs, W);
-        else if (rad == 3) fft_stage<R, 3>(s, P.N, P.M, P.lgM, lgNs, W);
-        else if (rad == 9) fft_stage<R, 9>(s, P.N, P.M, P.lgM, lgNs, W);
-        else fft_stage<R, 2>(s, P.N, P.M, P.lgM, lgNs, W);
+        if (rad == 8) fft_stage<R, 8>(s, P.N, P.M, P.lgM, lgNs, P.lds_stride, P.lds_shift, W);
+        else if (rad == 4) fft_stage<R, 4>(s, P.N, P.M, P.lgM, lgNs, P.lds_stride, P.lds_shift, W);
+        else if (rad == 3) fft_stage<R, 3>(s, P.N, P.M, P.lgM, lgNs, P.lds_stride, P.lds_shift, W);
+        else if (rad == 9) fft_stage<R, 9>(s, P.N, P.M, P.lgM, lgNs, P.lds_stride, P.lds_shift, W);
+        else fft_stage<R, 2>(s, P.N, P.M, P.lgM, lgNs, P.lds_stride, P.lds_shift, W);
         lgNs += rad == 8 ? 3 : rad == 4 ? 2 : 1;
     }
 }
@@ -263,7 +272,7 @@ __global__ __launch_bounds__(kFftThreads, 4) void k_fft_cols(typename Cplx<R>::T
 #pragma unroll
     for (int r = 0; r < EPT; ++r) {
         int idx = threadIdx.x + r * kFftThreads;
-        if (idx < M * N) s[fft_lds_idx(idx >> P.lgM, idx & (M - 1), M)] = v[r];
+        if (idx < M * N) s[fft_lds_idx(idx >> P.lgM, idx & (M - 1), P.lds_stride, P.lds_shift)] = v[r];
     }
     __syncthreads();
     fft_lds<R>(s, P, tw + P.w_off);
@@ -272,7 +281,7 @@ __global__ __launch_bounds__(kFftThreads, 4) void k_fft_cols(typename Cplx<R>::T
         int c = idx & (M - 1), k1 = idx >> P.lgM;
         int a = k1 * (t * M + c);          // < (N1 - 1) (N2 - 1) < Nf
         C2 w = CMul<R>::mul(hi[a >> 8], lo[a & 255]);
-        base[(long long)k1 * P.N2 + c] = CMul<R>::mul(s[fft_lds_idx(k1, c, M)], w);
+        base[(long long)k1 * P.N2 + c] = CMul<R>::mul(s[fft_lds_idx(k1, c, P.lds_stride, P.lds_shift)], w);
     }
 }
 
@@ -306,7 +315,7 @@ __global__ __launch_bounds__(kFftThreads, 4) void k_fft_rows(const typename Cplx
 #pragma unroll
         for (int r = 0; r < HALF; ++r) {
             int idx = threadIdx.x + (h * HALF + r) * kFftThreads;
-            if (idx < M * N) s[fft_lds_idx(idx & (N - 1), idx >> P.lgN, M)] = v[r];
+            if (idx < M * N) s[fft_lds_idx(idx & (N - 1), idx >> P.lgN, P.lds_stride, P.lds_shift)] = v[r];
         }
     }
     __syncthreads();
@@ -319,7 +328,7 @@ __global__ __launch_bounds__(kFftThreads, 4) void k_fft_rows(const typename Cplx
         int q = t * M + c + k2 * P.N1;
         int d = q - ci.y;                  // mlo in (-Nf, Nf)
         d = d < 0 ? d + nf : (d >= nf ? d - nf : d);
-        if (d <= ci.z) dst[(long long)k2 * P.N1 + c] = s[fft_lds_idx(k2, c, M)];
+        if (d <= ci.z) dst[(long long)k2 * P.N1 + c] = s[fft_lds_idx(k2, c, P.lds_stride, P.lds_shift)];
     }
 }
 
