@@ -118,3 +118,34 @@ def test_autotuned_plan_and_multi_batch(device):
     mb.close()
     for p in plans:
         p.close()
+
+
+@pytest.mark.parametrize("case", [
+    # one asymptotic box of Nf = 9 * 2^10 (r in [100, 300], k in [100, 170])
+    ("9216", 100.0, 300.0, 100.0, 170.0),
+    # the largest class at Nf = 9 * 2^12 (r, k in [0, 250])
+    ("36864", 0.0, 250.0, 0.0, 250.0),
+])
+@pytest.mark.parametrize("tol,precision", [(1e-8, "fp64"), (1e-6, "mixed")])
+def test_fft_lengths_nine_times_power_of_two(device, case, tol, precision):
+    """FFT lengths 9 * 2^a (the two-pass FFT kernels' radix-9 stage; the
+    plan's largest asymptotic box gets that length for these node ranges)
+    against the oracle's FP64 direct sums on sampled rows."""
+    import torch
+    import nufht_amd as na
+    name, ra, rb, ka, kb = case
+    s = W.Stream(7, 611)
+    n = 50_000
+    r = np.sort(s.uniform(n, ra, rb))
+    k = np.sort(s.uniform(n, ka, kb))
+    cs = s.normal(n * 3).reshape(3, n).T.copy(order="F")
+    rows = np.unique((W.Stream(9, 611).uniform(64) * k.size).astype(np.int32))
+    ref = oracle.direct_rows(3.0, r, cs, k, rows)
+    c = torch.as_tensor(np.asfortranarray(cs).T.copy(), device=device).t()
+    say(f"Nf {name} case: nu=3 M=N=5e4 B=3 tol={tol} {precision}")
+    p = na.Plan(3.0, r, k, tol=tol, precision=precision)
+    g = p.forward(c).cpu().numpy()
+    err = rel_l2(g[rows], ref)
+    say(f"  rel l2 {err:.2e}")
+    assert err < tol
+    p.close()
