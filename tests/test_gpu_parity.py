@@ -131,6 +131,8 @@ DENSE_CASES = [
     (2.0, 30000, 30000, 800.0, 60.0, 3, 1e-8, "fp64"),
     (1.0, 40000, 30000, 1000.0, 50.0, 16, 1e-6, "mixed"),
     (3.0, 30000, 30000, 300.0, 300.0, 5, 1e-6, "mixed"),
+    # kernel width 13 (tol 1e-11): the widest the matrix-core spread takes
+    (2.0, 30000, 30000, 800.0, 60.0, 4, 1e-11, "fp64"),
     # more than 16 columns: several column chunks, the last one partial
     (1.0, 30000, 30000, 1000.0, 50.0, 20, 1e-8, "fp64"),
     (2.0, 30000, 30000, 800.0, 60.0, 33, 1e-6, "mixed"),
