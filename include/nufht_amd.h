@@ -106,8 +106,33 @@ typedef struct {
      * multiply-adds */
     long long spread_sc_valu, spread_sc_mfma;
     long long interp_tc_valu, interp_tc_mfma;
+    /* the roofline model's fine grid per asymptotic box, nf_b = 2 sigma X S / pi
+     * + kernel_width + 1 (sigma = 2; X, S the box's r and k half-widths),
+     * weighted by the box's channel count C_b: sum C_b nf_b and
+     * sum C_b nf_b log2 nf_b */
+    double asy_fine, asy_fine_nlogn;
+    /* bytes the uniform FFTs move per call: for the two-pass kernels, pass 1
+     * reads a channel's 2L + 1 spread positions and writes its Nf points,
+     * pass 2 reads Nf and writes the frequencies interpolation reads; for
+     * rocFFT classes one read and one write of Nf points */
+    double fft_bytes;
+    /* the roofline model's algorithmic work of one call: FP64 flops, FP32
+     * flops, HBM bytes.  Direct sums 2 batch flops per (target, source) pair
+     * (Bessel evaluation not counted); local expansions 2 (K_loc + 1) batch
+     * per source and per target; per asymptotic box with C channels,
+     * 2 w C per source (spread), 4 w C per target (interpolation) and
+     * 2.5 C nf_b log2 nf_b (FFT), FP32 in mixed precision; bytes 8 (m + n)
+     * (1 + batch) plus 16 C nf_b per box (8 in mixed precision).  The
+     * floor of one call on MI355X is max(f64 / 78.6e12 + f32 / 157.3e12,
+     * bytes / 8e12) seconds. */
+    double model_f64, model_f32, model_bytes;
+    /* the direct-sum and local-expansion shares of model_f64 */
+    double model_dir, model_loc;
 } NufhtPlanStats;
 int nufht_amd_plan_stats(const NufhtPlan *plan, int batch, NufhtPlanStats *st);
+/* The same statistics summed over the transforms of a multi-transform batch
+ * (kernel_width, K_asy, K_loc and z_split are the first transform's). */
+int nufht_multi_stats(const NufhtMulti *mt, NufhtPlanStats *st);
 
 /* ---- launch-geometry knobs and the autotuner -----------------------------------
  * Choices that change which kernels run and how work is cut into items, never

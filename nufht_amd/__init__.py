@@ -82,7 +82,9 @@ class PlanStats(ctypes.Structure):
          ("K_asy", ctypes.c_int), ("K_loc", ctypes.c_int), ("z_split", ctypes.c_double),
          ("fft_classes", ctypes.c_longlong)] + \
         [(n, ctypes.c_longlong) for n in (
-            "spread_sc_valu", "spread_sc_mfma", "interp_tc_valu", "interp_tc_mfma")]
+            "spread_sc_valu", "spread_sc_mfma", "interp_tc_valu", "interp_tc_mfma")] + \
+        [(n, ctypes.c_double) for n in ("asy_fine", "asy_fine_nlogn", "fft_bytes", "model_f64",
+                                        "model_f32", "model_bytes", "model_dir", "model_loc")]
 
 
 class Knobs(ctypes.Structure):
@@ -109,7 +111,7 @@ EXPORTS = [
     "nufht_amd_plan_stats", "nufht_amd_autotune", "nufht_amd_version",
     "nufht_amd_last_error", "nufht_amd_get_knobs", "nufht_amd_set_knobs",
     "nufht_multi_autotune", "nufht_multi_get_knobs", "nufht_amd_plan_cache",
-    "nufht_amd_write_table_cache",
+    "nufht_amd_write_table_cache", "nufht_multi_stats",
 ]
 
 
@@ -163,6 +165,7 @@ def lib():
     L.nufht_amd_set_knobs.argtypes = [PP, ctypes.POINTER(Knobs)]
     L.nufht_multi_autotune.argtypes = [vp, vp]
     L.nufht_multi_get_knobs.argtypes = [vp, ctypes.POINTER(Knobs)]
+    L.nufht_multi_stats.argtypes = [vp, ctypes.POINTER(PlanStats)]
     L.nufht_amd_plan_cache.argtypes = [i, ctypes.POINTER(ctypes.c_longlong)]
     L.nufht_amd_write_table_cache.argtypes = [ctypes.c_char_p]
     L.nufht_amd_version.restype = ctypes.c_char_p
@@ -221,6 +224,19 @@ def _stream_handle(stream=None) -> Optional[int]:
     if stream is None:
         stream = torch.cuda.current_stream()
     return stream.cuda_stream
+
+
+def _check_out(t, shape, device, what: str):
+    """A destination buffer: float64, on `device`, contiguous, of `shape`
+    (the column-major (rows, B) data viewed as a (B, rows) tensor)."""
+    import torch
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_cuda:
+        raise ValueError(f"{what} must be a float64 CUDA (HIP) tensor")
+    if t.device != device:
+        raise ValueError(f"{what} is on {t.device}, the inputs on {device}")
+    if tuple(t.shape) != tuple(shape) or not t.is_contiguous():
+        raise ValueError(f"{what} must be a contiguous tensor of shape {tuple(shape)}, "
+                         f"got {tuple(t.shape)}{'' if t.is_contiguous() else ' (strided)'}")
 
 
 def _colmajor(t, rows: int):
@@ -309,6 +325,8 @@ class Plan:
         import torch
         c, ldc = _colmajor(cs, self.m)
         B = c.shape[0]
+        if out is not None:
+            _check_out(out, (B, self.n), c.device, "out")
         g = out if out is not None else torch.empty((B, self.n), dtype=torch.float64,
                                                     device=c.device)
         rc = lib().nufht_batch_device(ctypes.byref(self._plan), c.data_ptr(), ldc, g.data_ptr(),
@@ -416,6 +434,13 @@ class MultiBatch:
         gs_list[k]: contiguous (B_k, n_k) outputs."""
         L = lib()
         K = len(self.plans)
+        if len(cs_list) != K or len(gs_list) != K:
+            raise ValueError(f"expected {K} coefficient and {K} output tensors, got "
+                             f"{len(cs_list)} and {len(gs_list)}")
+        dev = getattr(cs_list[0], "device", None) if K else None
+        for q, (p, b, c, g) in enumerate(zip(self.plans, self.batches, cs_list, gs_list)):
+            _check_out(c, (b, p.m), dev, f"cs_list[{q}]")
+            _check_out(g, (b, p.n), dev, f"gs_list[{q}]")
         cp = (ctypes.c_void_p * K)(*[c.data_ptr() for c in cs_list])
         gp = (ctypes.c_void_p * K)(*[g.data_ptr() for g in gs_list])
         lc = (ctypes.c_int * K)(*[p.m for p in self.plans])
@@ -429,6 +454,13 @@ class MultiBatch:
         _check(lib().nufht_multi_autotune(self._h, _stream_handle(stream)),
                "nufht_multi_autotune")
         return self.knobs()
+
+    def stats(self) -> Dict[str, float]:
+        """Plan statistics summed over the batch's transforms (see
+        ``Plan.stats``)."""
+        st = PlanStats()
+        _check(lib().nufht_multi_stats(self._h, ctypes.byref(st)), "nufht_multi_stats")
+        return {f: getattr(st, f) for f, _ in PlanStats._fields_}
 
     def knobs(self) -> Dict[str, float]:
         k = Knobs()

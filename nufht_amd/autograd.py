@@ -91,8 +91,9 @@ class NufhtOperator:
         kh = np.ascontiguousarray(k.detach().to("cpu", torch.float64).numpy())
         key = (rh.tobytes(), kh.tobytes())
         if self._plan is None or key != self._key:
-            if self._plan is not None:
-                self._plan.close()
+            # the previous plan is not closed here: a recorded graph node may
+            # still hold it for its backward; it is freed when the last
+            # reference (this operator's or a node's) goes away
             self._plan = Plan(self.nu, rh, kh, tol=self.tol, precision=self.precision,
                               **self.plan_kw)
             self._key = key
@@ -113,9 +114,10 @@ class NufhtOperator:
         return restore(g)
 
     def close(self):
-        if self._plan is not None:
-            self._plan.close()
-            self._plan = None
+        """Drops the operator's plan (graph nodes that still hold it keep it
+        alive until their backward has run)."""
+        self._plan = None
+        self._key = None
 
 
 def nufht(nu: float, r: torch.Tensor, cs: torch.Tensor, k: torch.Tensor, tol: float = 1e-8,

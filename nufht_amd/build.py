@@ -47,18 +47,42 @@ def ext_path() -> str:
     return os.path.join(HERE, "_pynufht" + suffix)
 
 
+def _jobs() -> int:
+    n = os.environ.get("MAX_JOBS")
+    n = int(n) if n and n.isdigit() else (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
+# device translation units: (source, object, extra defines); the asymptotic
+# stage is compiled once per grid precision and exponent bound so that its
+# kernel instantiations build in parallel
+def _hip_units():
+    units = [("engine.hip", "engine.o", [])]
+    for rname, tag in (("double", "f64"), ("float", "f32")):
+        units.append(("fft_stage.hip", f"fft_{tag}.o", [f"-DNUFHT_ASY_R={rname}"]))
+        for em in (6, 10, 14, 22):
+            units.append(("asy_stage.hip", f"asy_{tag}_e{em}.o",
+                          [f"-DNUFHT_ASY_R={rname}", f"-DNUFHT_ASY_EM={em}"]))
+    return units
+
+
 def build(quiet: bool = True) -> None:
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(BUILD, exist_ok=True)
     hipcc = os.path.join(ROCM, "bin", "hipcc")
     incs = ["-I", INC, "-I", CSRC]
-    objs = []
+    cmds, objs = [], []
     for src in ("plan.c", "tables.c"):
         obj = os.path.join(BUILD, src.replace(".c", ".o"))
-        _run(["gcc", *C_FLAGS, *incs, "-c", os.path.join(CSRC, src), "-o", obj], quiet)
+        cmds.append(["gcc", *C_FLAGS, *incs, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    eng = os.path.join(BUILD, "engine.o")
-    _run([hipcc, *HIP_FLAGS, *incs, "-c", os.path.join(CSRC, "engine.hip"), "-o", eng], quiet)
-    objs.append(eng)
+    for src, obj, defs in _hip_units():
+        obj = os.path.join(BUILD, obj)
+        cmds.append([hipcc, *HIP_FLAGS, *defs, *incs, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    with ThreadPoolExecutor(max_workers=_jobs()) as ex:
+        for f in [ex.submit(_run, c, quiet) for c in cmds]:
+            f.result()
     tmp = lib_path() + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs,
           "-L", os.path.join(ROCM, "lib"), "-lrocfft", "-lamdhip64",

@@ -25,7 +25,7 @@
 
 #include "internal.h"
 #include "kernels.h"
-#include "fft.h"
+#include "program.h"
 
 using namespace nufht_dev;
 
@@ -62,9 +62,6 @@ extern "C" const char *nufht_amd_version(void) { return "nufht-amd 0.1 gfx950"; 
 
 // ---------------------------------------------------------------------------
 // per-stage timing
-// (one kernel per stage, except asy_fft: the rocFFT launches of all classes)
-enum Stage { ST_ZERO, ST_PERM, ST_DIR, ST_LOCP, ST_LOCE, ST_GZERO, ST_SPREAD, ST_SPREAD_MF,
-             ST_FFT, ST_INTERP, ST_INTERP_MF, ST_GRAD, ST_COUNT };
 static const char *kStageNames[ST_COUNT] = {"zero", "permute", "direct", "loc_project",
                                             "loc_eval", "asy_grid_zero", "asy_spread",
                                             "asy_spread_mfma", "asy_fft", "asy_interp",
@@ -76,28 +73,13 @@ static std::vector<PendingEv> g_prof_pending;
 static double g_prof_ms[ST_COUNT];
 static long long g_prof_n[ST_COUNT];
 
-struct StageTimer {
-    int stage;
-    hipStream_t st;
-    hipEvent_t a = nullptr;
-    StageTimer(int s, hipStream_t stream) : stage(s), st(stream)
-    {
-        if (g_prof_on) {
-            hipEventCreate(&a);
-            hipEventRecord(a, st);
-        }
-    }
-    ~StageTimer()
-    {
-        if (a) {
-            hipEvent_t b;
-            hipEventCreate(&b);
-            hipEventRecord(b, st);
-            std::lock_guard<std::mutex> g(g_prof_mu);
-            g_prof_pending.push_back({a, b, stage});
-        }
-    }
-};
+bool nufht_dev::prof_enabled() { return g_prof_on; }
+
+void nufht_dev::prof_push(hipEvent_t a, hipEvent_t b, int stage)
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    g_prof_pending.push_back({a, b, stage});
+}
 
 extern "C" int nufht_amd_profile(int enable)
 {
@@ -246,9 +228,6 @@ static const KernelTable &kernel_table(int kw, double beta)
 // largest argument covered by a direct-sum Bessel table (unit intervals)
 static const int kJTabMax = 8192;
 
-// sources per spread item before a position tile is split across items
-static const int kSpreadSplit = 2048;
-static const int kDenseSplit = 0;      // 0: from the plan's size
 // sources per grid cell from which a box uses the split-K spread kernel
 static const double kDenseCell = 4.0;
 // (NUFHT_AMD_DENSE_CELL overrides the threshold)
@@ -256,36 +235,8 @@ static double g_dense_cell = [] {
     const char *e = getenv("NUFHT_AMD_DENSE_CELL");
     return e ? atof(e) : kDenseCell;
 }();
+double nufht_dev::knobs_default_dense_cell() { return g_dense_cell; }
 
-// Launch-geometry choices of a program (NufhtAmdKnobs without precision).
-struct Knobs {
-    double dense_cell = g_dense_cell;
-    int dense_split = kDenseSplit;
-    int spread_split = kSpreadSplit;
-    int interp_mfma = 1;
-    int own_fft = 1;
-    int spread_sched = -1;   // matrix-core spread schedule: -1 automatic, 0 unified, 1 split
-    bool operator==(const Knobs &o) const
-    {
-        return dense_cell == o.dense_cell && dense_split == o.dense_split &&
-               spread_split == o.spread_split && interp_mfma == o.interp_mfma &&
-               own_fft == o.own_fft && spread_sched == o.spread_sched;
-    }
-    bool operator!=(const Knobs &o) const { return !(*this == o); }
-};
-
-// FFT lengths: 2^a and 3 2^a (prebuilt rocFFT kernels), and above 4096 (the
-// two-pass FFT kernels' range) 9 2^a
-static int fft_length(long long need)
-{
-    long long best = 1LL << 40;
-    for (long long p = 8; p < (1LL << 34); p <<= 1) {
-        if (p >= need) { best = std::min(best, p); break; }
-        if (3 * p >= need) best = std::min(best, 3 * p);
-        if (9 * p >= need && 9 * p > 4096) best = std::min(best, 9 * p);
-    }
-    return (int)best;
-}
 
 // Two-pass factorisation Nf = N1 N2 for the FFT kernels (Nf = 2^a, 3 2^a or
 // 9 2^a above 4096; the factor 3 or 9 goes into N1), sequences per block and
@@ -367,67 +318,7 @@ struct DevPlan {
     int *d_perm_r = nullptr, *d_perm_w = nullptr;
 };
 
-struct FftClass {
-    int nf = 0;
-    long long batch = 0, off = 0;
-    bool own = false;          // two-pass FFT kernels (else rocFFT, out of place)
-    FftPass p1, p2;
-    size_t lds1 = 0, lds2 = 0;
-    rocfft_plan plan = nullptr;
-    rocfft_execution_info info = nullptr;
-    void *work = nullptr;
-    size_t work_bytes = 0;
-};
 
-struct Program {
-    Knobs knobs;      // set before build_program
-    bool spread_split = false;   // matrix-core spread schedule (resolved from knobs)
-    int prec = 0;     // 0 FP64, 2 mixed
-    int emax = 0;
-    std::vector<TfDesc> tfs;
-    TfDesc *d_tfs = nullptr;
-    std::vector<ZeroItem> zero;
-    ZeroItem *d_zero = nullptr;
-    std::vector<DirItem> dir;
-    DirItem *d_dir = nullptr;
-    std::vector<int2> dir_rng;        // source ranges {j0, count} of the direct items
-    int2 *d_dir_rng = nullptr;
-    std::vector<LocBoxD> lbox;
-    LocBoxD *d_lbox = nullptr;
-    std::vector<LocItem> lproj;
-    LocItem *d_lproj = nullptr;
-    std::vector<LocItem> leval1;      // targets covered by one local box
-    LocItem *d_leval1 = nullptr;
-    std::vector<LocEvalItem> leval;   // targets covered by several
-    LocEvalItem *d_leval = nullptr;
-    std::vector<int> lev_boxes;       // box lists of the evaluation items
-    int *d_lev_boxes = nullptr;
-    int loc_bmax = 16;                // columns per evaluation item (16 or 32)
-    double *d_coef = nullptr;
-    long long ncoef = 0;
-    int loc_ch = 1024, loc_tpb = 256, loc_nlt = 1;
-    size_t loc_lds_e = 0;
-    std::vector<AsyBoxD> abox;
-    AsyBoxD *d_abox = nullptr;
-    std::vector<AsyItem> aspread, ainterp, azero, adense, aimf;
-    std::vector<int2> arange;
-    std::vector<SpreadRange> adrange;
-    AsyItem *d_aspread = nullptr, *d_ainterp = nullptr, *d_azero = nullptr, *d_adense = nullptr;
-    AsyItem *d_aimf = nullptr;
-    int2 *d_arange = nullptr;
-    SpreadRange *d_adrange = nullptr;
-    int interp_tmax = 16;
-    void *d_grid = nullptr, *d_grid2 = nullptr;   // spread grid; transformed grid
-    void *d_fftw = nullptr;                        // FFT twiddle tables
-    std::vector<int4> fft_chinfo;                  // per channel of own FFT classes: L, mlo, span
-    int4 *d_fft_chinfo = nullptr;
-    long long grid_elems = 0;
-    std::vector<FftClass> fft;
-    double *d_ft = nullptr, *d_cheb = nullptr, *d_jtab = nullptr;
-    long long jtab_total = 0;
-    NufhtPlanStats stats;
-    bool desc_dirty = true;
-};
 
 // Identity of a plan's device state: order, options, sizes, device and
 // 128 bits of hash of the caller's r and k arrays.
@@ -452,7 +343,27 @@ struct EngineState {
     size_t in_cap = 0, out_cap = 0;
     double *d_tmp[6] = {nullptr};
     size_t tmp_cap[6] = {0};
+    // the last call's end on its stream: a call on another stream waits for
+    // it before rewriting the program's descriptors or the staging buffers
+    hipEvent_t done_ev = nullptr;
+    hipStream_t done_st = nullptr;
 };
+
+// Orders a call on `st` after the previous call on this state (a no-op on
+// the same stream).
+static int order_after_last(EngineState *E, hipStream_t st)
+{
+    if (E->done_ev && E->done_st != st) HIP_TRY(hipStreamWaitEvent(st, E->done_ev, 0));
+    return 0;
+}
+
+static int mark_done(EngineState *E, hipStream_t st)
+{
+    if (!E->done_ev) HIP_TRY(hipEventCreateWithFlags(&E->done_ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(E->done_ev, st));
+    E->done_st = st;
+    return 0;
+}
 
 static int ensure_buf(double **p, size_t *cap, size_t need)
 {
@@ -601,6 +512,7 @@ static void engine_destroy(EngineState *E)
     hipFree(E->dev.d_r); hipFree(E->dev.d_w); hipFree(E->dev.d_perm_r); hipFree(E->dev.d_perm_w);
     hipFree(E->d_in); hipFree(E->d_out);
     for (int i = 0; i < 6; ++i) hipFree(E->d_tmp[i]);
+    if (E->done_ev) hipEventDestroy(E->done_ev);
     delete E;
 }
 
@@ -780,6 +692,15 @@ static void add_dir_items(Program *P, int tf, int B, const std::vector<Box> &dir
     }
 }
 
+// Local expansions the local kernels run: K_loc + 1 terms in at most
+// kLocMaxNlt row tiles of 16, and a per-target scratch (loc_slots words) that
+// 64 targets per block keep within 64 KiB of LDS.
+constexpr int kLocSlotsMax = 128;
+static bool loc_fits(int order, int half, int K)
+{
+    return K >= 0 && (K + 16) / 16 <= kLocMaxNlt && loc_slots(order, half, K) <= kLocSlotsMax;
+}
+
 static int build_program(Program *P, const std::vector<TfInput> &in)
 {
     static std::once_flag fft_once;
@@ -842,6 +763,14 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 for (size_t q = 0; q < bx.loc.length; ++q) locb.push_back(bx.loc.data[q]);
             for (size_t q = 0; q < bx.asy.length; ++q) asyb.push_back(bx.asy.data[q]);
         }
+        // local boxes whose expansion exceeds what the local kernels hold (K_loc
+        // up to 16 kLocMaxNlt - 1; per-target scratch of at most kLocSlotsMax
+        // words, which high orders pass: order 60 at tol 1e-8 has K_loc = 207)
+        // are summed directly instead
+        if (!locb.empty() && !loc_fits(T.order, T.half, T.K_loc)) {
+            dirb.insert(dirb.end(), locb.begin(), locb.end());
+            locb.clear();
+        }
         // direct-sum Bessel table over [0, max w r of the direct boxes]
         double xdir = 0.0;
         for (auto &b : dirb)
@@ -854,7 +783,11 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
             if (ni <= 0 || nj <= 0) continue;
             P->stats.pairs_dir += (long long)ni * nj;
             P->stats.n_dir++;
+            P->stats.model_f64 += 2.0 * B * (double)ni * (double)nj;
+            P->stats.model_dir += 2.0 * B * (double)ni * (double)nj;
         }
+        // compulsory traffic: the nodes, the coefficients and the outputs
+        P->stats.model_bytes += 8.0 * ((double)plan->m + plan->n) * (1.0 + B);
         add_dir_items(P, tf, B, dirb);
         if (!locb.empty()) {
             kloc_max = std::max(kloc_max, T.K_loc);
@@ -884,6 +817,8 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
             P->stats.n_loc++;
             P->stats.loc_src += nj;
             P->stats.loc_tgt += ni;
+            P->stats.model_f64 += 2.0 * (T.K_loc + 1) * B * (double)(ni + nj);
+            P->stats.model_loc += 2.0 * (T.K_loc + 1) * B * (double)(ni + nj);
         }
         int E2 = 2 * (T.K_asy + 1);
         int tb = 1;
@@ -903,9 +838,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
             if (S < Smin) S = Smin;
             const double sigma = 2.0;
             A.h = M_PI / (sigma * S);
-            A.L = (int)std::ceil(X / A.h) + T.kw / 2 + 2;
-            long long Ng = 2LL * A.L + 1;
-            A.Nf = fft_length(std::max<long long>((long long)(sigma * Ng), 2LL * T.kw + 8));
+            RC_TRY(nufht_asy_grid(X, A.h, T.kw, &A.L, &A.Nf));
             A.delta = 2.0 * M_PI / ((double)A.Nf * A.h);
             A.a1 = 0.5 * T.kw * A.h;
             A.a2 = 0.5 * T.kw * A.delta;
@@ -916,6 +849,13 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
             P->stats.asy_src += nj;
             P->stats.asy_tgt += ni;
             P->stats.asy_channels += (long long)E2 * B;
+            const double nfb = 4.0 * X * S / M_PI + T.kw + 1.0;
+            P->stats.asy_fine += (double)E2 * B * nfb;
+            P->stats.asy_fine_nlogn += (double)E2 * B * nfb * std::log2(nfb);
+            const double C = (double)E2 * B;
+            const double fn = 2.0 * T.kw * C * nj + 4.0 * T.kw * C * ni + 2.5 * C * nfb * std::log2(nfb);
+            (mixed ? P->stats.model_f32 : P->stats.model_f64) += fn;
+            P->stats.model_bytes += (mixed ? 8.0 : 16.0) * C * nfb;
         }
         P->tfs.push_back(T);
     }
@@ -952,6 +892,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 if (mhi - mlo + 1 >= A.Nf) { mlo = 0; mhi = A.Nf - 1; }
                 int4 info = make_int4(A.L, mlo, mhi - mlo, 0);
                 for (long long k = 0; k < ch; ++k) P->fft_chinfo.push_back(info);
+                P->stats.fft_bytes += (double)ch * (mixed ? 8.0 : 16.0) *
+                                      ((2.0 * A.L + 1.0) + 2.0 * A.Nf + (mhi - mlo + 1.0));
+            } else {
+                P->stats.fft_bytes += (double)ch * (mixed ? 8.0 : 16.0) * 2.0 * A.Nf;
             }
         }
         P->fft.push_back(c);
@@ -1110,6 +1054,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     if (!P->lbox.empty()) {
         P->loc_nlt = (kloc_max + 16) / 16;
         if (P->loc_nlt > kLocMaxNlt) {
+            // (loc_fits sends such boxes to the direct sums: not reached)
             nufht_set_error("local expansion with K_loc = %d exceeds the supported %d", kloc_max,
                             16 * kLocMaxNlt - 1);
             return -540;
@@ -1218,18 +1163,6 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         } else {
             RC_TRY(upload((double **)&P->d_fftw, tw));
         }
-        static bool attr_done = false;
-        if (!attr_done) {
-            hipFuncSetAttribute((const void *)k_fft_cols<double>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            hipFuncSetAttribute((const void *)k_fft_rows<double>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            hipFuncSetAttribute((const void *)k_fft_cols<float>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            hipFuncSetAttribute((const void *)k_fft_rows<float>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            attr_done = true;
-        }
     }
     for (auto &c : P->fft) {
         if (c.own) continue;
@@ -1265,75 +1198,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
 template <typename R, int EM>
 static void launch_asy(Program *P, hipStream_t st)
 {
-    typedef typename Cplx<R>::T G;
-    if (!P->azero.empty()) {
-        StageTimer tm(ST_GZERO, st);
-        hipLaunchKernelGGL((k_grid_zero<R>), dim3((unsigned)P->azero.size()), dim3(256), 0, st,
-                           P->d_tfs, P->d_abox, P->d_azero, (G *)P->d_grid);
-    }
-    if (!P->aspread.empty()) {
-        StageTimer tm(ST_SPREAD, st);
-        hipLaunchKernelGGL((k_asy_spread<R, EM>), dim3((unsigned)P->aspread.size()), dim3(256),
-                           0, st, P->d_tfs, P->d_abox, P->d_aspread, P->d_arange,
-                           (G *)P->d_grid, P->d_ft);
-    }
-    if (!P->adense.empty()) {
-        StageTimer tm(ST_SPREAD_MF, st);
-        static std::once_flag attr_once;
-        std::call_once(attr_once, []() {
-            hipFuncSetAttribute((const void *)k_asy_spread_mfma<R, EM, true>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)SpreadSched<true>::template lds<R, EM>());
-            hipFuncSetAttribute((const void *)k_asy_spread_mfma<R, EM, false>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)SpreadSched<false>::template lds<R, EM>());
-        });
-        const size_t lds_split = SpreadSched<true>::template lds<R, EM>();
-        const size_t lds_unified = SpreadSched<false>::template lds<R, EM>();
-        if (P->spread_split)
-            hipLaunchKernelGGL((k_asy_spread_mfma<R, EM, true>), dim3((unsigned)P->adense.size()),
-                               dim3(SpreadSched<true>::threads), lds_split, st, P->d_tfs,
-                               P->d_abox, P->d_adense, P->d_adrange, (G *)P->d_grid, P->d_ft);
-        else
-            hipLaunchKernelGGL((k_asy_spread_mfma<R, EM, false>), dim3((unsigned)P->adense.size()),
-                               dim3(SpreadSched<false>::threads), lds_unified, st, P->d_tfs,
-                               P->d_abox, P->d_adense, P->d_adrange, (G *)P->d_grid, P->d_ft);
-    }
-    {
-        StageTimer tm(ST_FFT, st);
-        for (auto &c : P->fft) {
-            G *in = (G *)P->d_grid + c.off, *out = (G *)P->d_grid2 + c.off;
-            if (c.own) {
-                const G *tw = (const G *)P->d_fftw;
-                hipLaunchKernelGGL((k_fft_cols<R>), dim3((unsigned)(c.batch * c.p1.tiles)),
-                                   dim3(kFftThreads), c.lds1, st, in, c.p1, tw,
-                                   (const int4 *)P->d_fft_chinfo);
-                hipLaunchKernelGGL((k_fft_rows<R>), dim3((unsigned)(c.batch * c.p2.tiles)),
-                                   dim3(kFftThreads), c.lds2, st, (const G *)in, out, c.p2, tw,
-                                   (const int4 *)P->d_fft_chinfo);
-                continue;
-            }
-            rocfft_execution_info_set_stream(c.info, st);
-            void *ib = (void *)in, *ob = (void *)out;
-            rocfft_execute(c.plan, &ib, &ob, c.info);
-        }
-    }
-    if (!P->ainterp.empty()) {
-        StageTimer tm(ST_INTERP, st);
-        int tmx = P->interp_tmax;
-        size_t lds = interp_lds_off(tmx, 2, sizeof(R)) * sizeof(double);
-        hipLaunchKernelGGL((k_asy_interp<R, EM>), dim3((unsigned)P->ainterp.size()), dim3(256),
-                           lds, st, P->d_tfs, P->d_abox, P->d_ainterp, (const G *)P->d_grid2,
-                           P->d_ft, tmx);
-    }
-    if (!P->aimf.empty()) {
-        StageTimer tm(ST_INTERP_MF, st);
-        // FP64 products for both grid precisions: the FP32 matrix-core form of
-        // this kernel measured 1.7x slower on gfx950
-        hipLaunchKernelGGL((k_asy_interp_mfma<R, double, EM>), dim3((unsigned)P->aimf.size()),
-                           dim3(64), 0, st, P->d_tfs, P->d_abox, P->d_aimf,
-                           (const G *)P->d_grid2, P->d_ft);
-    }
+    asy_grid_zero<R>(P, st);
+    asy_spread<R, EM>(P, st);
+    asy_fft<R>(P, st);
+    asy_interp<R, EM>(P, st);
 }
 
 template <typename R>
@@ -1347,25 +1215,27 @@ static void launch_asy_e(Program *P, hipStream_t st)
     }
 }
 
-template <int NLT> static void launch_loc_proj_n(Program *P, hipStream_t st)
+template <int NLT> static void launch_loc_proj_n(Program *P, hipStream_t st, double *coef)
 {
     hipLaunchKernelGGL(k_loc_proj<NLT>, dim3((unsigned)P->lproj.size()), dim3(256),
-                       LocProjLds<NLT>::bytes, st, P->d_tfs, P->d_lbox, P->d_lproj, P->d_coef);
+                       LocProjLds<NLT>::bytes, st, P->d_tfs, P->d_lbox, P->d_lproj, coef);
 }
 
-static void launch_loc_proj(Program *P, hipStream_t st)
+static void launch_loc_proj(Program *P, hipStream_t st, double *coef)
 {
     switch (P->loc_nlt) {
-    case 1: launch_loc_proj_n<1>(P, st); break;
-    case 2: launch_loc_proj_n<2>(P, st); break;
-    case 3: launch_loc_proj_n<3>(P, st); break;
-    case 4: launch_loc_proj_n<4>(P, st); break;
-    case 5: launch_loc_proj_n<5>(P, st); break;
-    default: launch_loc_proj_n<6>(P, st); break;
+    case 1: launch_loc_proj_n<1>(P, st, coef); break;
+    case 2: launch_loc_proj_n<2>(P, st, coef); break;
+    case 3: launch_loc_proj_n<3>(P, st, coef); break;
+    case 4: launch_loc_proj_n<4>(P, st, coef); break;
+    case 5: launch_loc_proj_n<5>(P, st, coef); break;
+    default: launch_loc_proj_n<6>(P, st, coef); break;
     }
 }
 
-static int run_program(Program *P, hipStream_t st)
+// coef: the local-expansion moments' buffer (coef, or the caller's
+// scratch of at least P->ncoef doubles); it is cleared before use.
+static int run_program(Program *P, hipStream_t st, double *coef)
 {
     if (P->desc_dirty) {
         HIP_TRY(hipMemcpyAsync(P->d_tfs, P->tfs.data(), P->tfs.size() * sizeof(TfDesc),
@@ -1378,28 +1248,28 @@ static int run_program(Program *P, hipStream_t st)
                            P->d_zero);
         if (P->ncoef)
             hipLaunchKernelGGL(k_fill0, dim3((unsigned)std::min<long long>(1024, (P->ncoef + 255) / 256)),
-                               dim3(256), 0, st, P->d_coef, P->ncoef);
+                               dim3(256), 0, st, coef, P->ncoef);
     }
     if (!P->lproj.empty()) {
         {
             StageTimer tm(ST_LOCP, st);
-            launch_loc_proj(P, st);
+            launch_loc_proj(P, st, coef);
         }
         {
             StageTimer tm(ST_LOCE, st);
             if (!P->leval1.empty())
                 hipLaunchKernelGGL(k_loc_eval1, dim3((unsigned)P->leval1.size()), dim3(P->loc_tpb),
-                                   P->loc_lds_e, st, P->d_tfs, P->d_lbox, P->d_leval1, P->d_coef,
+                                   P->loc_lds_e, st, P->d_tfs, P->d_lbox, P->d_leval1, coef,
                                    P->d_cheb);
             if (P->leval.empty()) {
             } else if (P->loc_bmax == 32)
                 hipLaunchKernelGGL(k_loc_eval<32>, dim3((unsigned)P->leval.size()),
                                    dim3(P->loc_tpb), P->loc_lds_e, st, P->d_tfs, P->d_lbox,
-                                   P->d_leval, P->d_lev_boxes, P->d_coef, P->d_cheb);
+                                   P->d_leval, P->d_lev_boxes, coef, P->d_cheb);
             else
                 hipLaunchKernelGGL(k_loc_eval<16>, dim3((unsigned)P->leval.size()),
                                    dim3(P->loc_tpb), P->loc_lds_e, st, P->d_tfs, P->d_lbox,
-                                   P->d_leval, P->d_lev_boxes, P->d_coef, P->d_cheb);
+                                   P->d_leval, P->d_lev_boxes, coef, P->d_cheb);
         }
     }
     if (!P->dir.empty()) {
@@ -1455,8 +1325,10 @@ static int exec_sorted(const NufhtPlan *plan, const double *d_cs, long long ld_c
     EngineState *E = engine_of(plan);
     Program *P;
     RC_TRY(get_program(plan, E, B, &P));
+    RC_TRY(order_after_last(E, st));
     set_io(P, 0, d_cs, ld_cs, d_gs, ld_gs);
-    return run_program(P, st);
+    RC_TRY(run_program(P, st, P->d_coef));
+    return mark_done(E, st);
 }
 
 static int gather_rows(const double *src, long long ld_src, double *dst, long long ld_dst,
@@ -1512,8 +1384,9 @@ extern "C" int nufht_batch_device(const NufhtPlan *plan, const double *d_cs, int
     if (ld_cs < plan->m || ld_gs < plan->n) return -401;
     EngineState *E = engine_of(plan);
     std::lock_guard<std::mutex> g(E->mu);
+    RC_TRY(order_after_last(E, (hipStream_t)stream));
     int rc = exec_device(plan, d_cs, ld_cs, d_gs, ld_gs, batch, (hipStream_t)stream);
-    if (g_prof_on) hipStreamSynchronize((hipStream_t)stream);
+    if (!rc) rc = mark_done(E, (hipStream_t)stream);
     return rc;
 }
 
@@ -1529,6 +1402,7 @@ extern "C" int nufht_engine_batch_host(const NufhtPlan *plan, const double *cs, 
     RC_TRY(ensure_buf(&E->d_in, &E->in_cap, (size_t)m * batch));
     RC_TRY(ensure_buf(&E->d_out, &E->out_cap, (size_t)n * batch));
     hipStream_t st = nullptr;
+    RC_TRY(order_after_last(E, st));
     if (h->perm_r) {
         std::vector<double> tmp((size_t)m * batch);
         for (int t = 0; t < batch; ++t)
@@ -1583,6 +1457,7 @@ extern "C" int nufht_adjoint_device(const NufhtPlan *plan, const double *d_ys, i
     std::lock_guard<std::mutex> g(E->mu);
     hipStream_t st = (hipStream_t)stream;
     RC_TRY(ensure_dev(plan, E));
+    RC_TRY(order_after_last(E, st));
     NufhtPlan *adj;
     RC_TRY(derived(plan, 0, &adj));
     EngineState *EA = engine_of(adj);
@@ -1605,7 +1480,7 @@ extern "C" int nufht_adjoint_device(const NufhtPlan *plan, const double *d_ys, i
             rc = exec_sorted(adj, ys, ldy, d_xs, ld_xs, batch, st);
         }
     }
-    if (g_prof_on) hipStreamSynchronize(st);
+    if (!rc) rc = mark_done(E, st);
     return rc;
 }
 
@@ -1648,6 +1523,7 @@ extern "C" int nufht_grad_nodes_device(const NufhtPlan *plan, const double *d_cs
     hipStream_t st = (hipStream_t)stream;
     int m = plan->m, n = plan->n, B = batch;
     RC_TRY(ensure_dev(plan, E));
+    RC_TRY(order_after_last(E, st));
     double sgn = (plan->nu < 0 && std::fabs(plan->nu - std::round(plan->nu)) < 1e-12 &&
                   (std::llround(plan->nu) & 1)) ? -1.0 : 1.0;
     // sorted copies of cs (m x B) and rho (n x B)
@@ -1709,7 +1585,7 @@ extern "C" int nufht_grad_nodes_device(const NufhtPlan *plan, const double *d_cs
         RC_TRY(scatter_rows(aux, n, d_grad_w, n, E->dev.d_perm_w, n, 1, st));
     }
     HIP_TRY(hipGetLastError());
-    if (g_prof_on) hipStreamSynchronize(st);
+    RC_TRY(mark_done(E, st));
     return 0;
 }
 
@@ -1749,7 +1625,41 @@ struct NufhtMulti {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     std::vector<const void *> gkey;
+    // sorted-order staging of transforms whose caller grids are not
+    // ascending: coefficients gathered in, outputs scattered back
+    std::vector<double *> st_in, st_out;
 };
+
+static void multi_destroy(NufhtMulti *mt)
+{
+    if (mt->gexec) hipGraphExecDestroy(mt->gexec);
+    if (mt->graph) hipGraphDestroy(mt->graph);
+    free_program(mt->P);
+    for (double *p : mt->st_in) hipFree(p);
+    for (double *p : mt->st_out) hipFree(p);
+    delete mt;
+}
+
+// One pass of the batch on `st`: row gathers of unsorted transforms, the
+// program, row scatters back to the callers' order.
+static int multi_run(NufhtMulti *mt, const double *const *d_cs, const int *ld_cs,
+                     double *const *d_gs, const int *ld_gs, hipStream_t st)
+{
+    for (size_t k = 0; k < mt->plans.size(); ++k)
+        if (mt->st_in[k]) {
+            const NufhtPlan *p = mt->plans[k];
+            RC_TRY(gather_rows(d_cs[k], ld_cs[k], mt->st_in[k], p->m,
+                               engine_of(p)->dev.d_perm_r, p->m, mt->B[k], st));
+        }
+    RC_TRY(run_program(mt->P, st, mt->P->d_coef));
+    for (size_t k = 0; k < mt->plans.size(); ++k)
+        if (mt->st_out[k]) {
+            const NufhtPlan *p = mt->plans[k];
+            RC_TRY(scatter_rows(mt->st_out[k], p->n, d_gs[k], ld_gs[k],
+                                engine_of(p)->dev.d_perm_w, p->n, mt->B[k], st));
+        }
+    return 0;
+}
 
 extern "C" int nufht_multi_create(NufhtMulti **out, int count, const NufhtPlan *const *plans,
                                   const int *batch)
@@ -1758,18 +1668,33 @@ extern "C" int nufht_multi_create(NufhtMulti **out, int count, const NufhtPlan *
     NufhtMulti *mt = new NufhtMulti();
     std::vector<TfInput> in;
     for (int k = 0; k < count; ++k) {
-        if (!plans[k] || !plans[k]->impl || batch[k] <= 0) { delete mt; return -400; }
+        if (!plans[k] || !plans[k]->impl || batch[k] <= 0) { multi_destroy(mt); return -400; }
         EngineState *E = engine_of(plans[k]);
         int rc = ensure_dev(plans[k], E);
-        if (rc) { delete mt; return rc; }
+        if (rc) { multi_destroy(mt); return rc; }
         mt->plans.push_back(plans[k]);
         mt->B.push_back(batch[k]);
         in.push_back({plans[k], batch[k]});
+        const NufhtImplHost *h = (const NufhtImplHost *)plans[k]->impl;
+        double *si = nullptr, *so = nullptr;
+        if (h->perm_r &&
+            hipMalloc((void **)&si, sizeof(double) * plans[k]->m * batch[k]) != hipSuccess)
+            si = nullptr, rc = -510;
+        if (!rc && h->perm_w &&
+            hipMalloc((void **)&so, sizeof(double) * plans[k]->n * batch[k]) != hipSuccess)
+            so = nullptr, rc = -510;
+        mt->st_in.push_back(si);
+        mt->st_out.push_back(so);
+        if (rc) {
+            nufht_set_error("multi-transform staging: out of device memory");
+            multi_destroy(mt);
+            return rc;
+        }
     }
     mt->P = new Program();
     mt->P->knobs = mt->knobs;
     int rc = build_program(mt->P, in);
-    if (rc) { free_program(mt->P); delete mt; return rc; }
+    if (rc) { multi_destroy(mt); return rc; }
     *out = mt;
     return 0;
 }
@@ -1800,19 +1725,19 @@ extern "C" int nufht_multi_execute(NufhtMulti *mt, const double *const *d_cs, co
     std::vector<const void *> key;
     for (size_t k = 0; k < mt->plans.size(); ++k) {
         const NufhtPlan *p = mt->plans[k];
-        NufhtImplHost *h = (NufhtImplHost *)p->impl;
-        if (h->perm_r || h->perm_w) {
-            nufht_set_error("multi-transform batches need ascending grids");
-            return -402;
-        }
+        if (!d_cs[k] || !d_gs[k]) return -400;
         if (ld_cs[k] < p->m || ld_gs[k] < p->n) return -401;
-        set_io(mt->P, (int)k, d_cs[k], ld_cs[k], d_gs[k], ld_gs[k]);
+        // unsorted transforms read and write the staging buffers
+        set_io(mt->P, (int)k, mt->st_in[k] ? mt->st_in[k] : d_cs[k],
+               mt->st_in[k] ? p->m : ld_cs[k], mt->st_out[k] ? mt->st_out[k] : d_gs[k],
+               mt->st_out[k] ? p->n : ld_gs[k]);
         key.push_back(d_cs[k]);
         key.push_back(d_gs[k]);
+        key.push_back((const void *)(intptr_t)ld_cs[k]);
+        key.push_back((const void *)(intptr_t)ld_gs[k]);
     }
     if (!use_graph || g_prof_on) {
-        int rc = run_program(mt->P, st);
-        if (g_prof_on) hipStreamSynchronize(st);
+        int rc = multi_run(mt, d_cs, ld_cs, d_gs, ld_gs, st);
         return rc;
     }
     if (mt->P->desc_dirty) {
@@ -1826,7 +1751,7 @@ extern "C" int nufht_multi_execute(NufhtMulti *mt, const double *const *d_cs, co
         hipStream_t cap;
         HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-        int rc = run_program(mt->P, cap);
+        int rc = multi_run(mt, d_cs, ld_cs, d_gs, ld_gs, cap);
         hipGraph_t gr = nullptr;
         hipError_t ce = hipStreamEndCapture(cap, &gr);
         hipStreamDestroy(cap);
@@ -1843,13 +1768,16 @@ extern "C" int nufht_multi_execute(NufhtMulti *mt, const double *const *d_cs, co
     return 0;
 }
 
+extern "C" int nufht_multi_stats(const NufhtMulti *mt, NufhtPlanStats *st)
+{
+    if (!mt || !mt->P || !st) return -400;
+    *st = mt->P->stats;
+    return 0;
+}
+
 extern "C" void nufht_multi_free(NufhtMulti *mt)
 {
-    if (!mt) return;
-    if (mt->gexec) hipGraphExecDestroy(mt->gexec);
-    if (mt->graph) hipGraphDestroy(mt->graph);
-    free_program(mt->P);
-    delete mt;
+    if (mt) multi_destroy(mt);
 }
 
 // ---------------------------------------------------------------------------

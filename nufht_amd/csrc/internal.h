@@ -40,6 +40,13 @@ int  nufht_tables_lookup(int nu_int, int j_idx, int k_slot, double *z, int *wimp
 double nufht_asy_error_bound(double nu, int K, double z);
 
 /* plan.c helpers used by the engine */
+/* Largest uniform grid (FFT length) of one asymptotic box: 2^30 points. */
+#define NUFHT_ASY_MAX_NF (1LL << 30)
+/* Grid of an asymptotic box with source half-width X and grid spacing h for
+ * an ES kernel of width kw: half-width L (positions -L..L) and FFT length
+ * Nf.  Returns 0, or -541 (with the error text set) when Nf would exceed
+ * NUFHT_ASY_MAX_NF. */
+int nufht_asy_grid(double X, double h, int kw, int *L, int *Nf);
 int nufht_build_partition(const double *r, int m, const double *w, int n,
                           double z, int max_levels, int min_dim_prod, int kloc,
                           Boxes *out);

@@ -225,6 +225,48 @@ int nufht_build_partition(const double *r, int m, const double *w, int n, double
     return 0;
 }
 
+/* FFT length for a grid of `need` points: the smallest 2^a or 3 2^a, or
+ * above 4096 (the two-pass FFT kernels' range) 9 2^a, that holds it. */
+static long long fft_length_for(long long need)
+{
+    long long best = -1;
+    for (long long p = 8; p <= NUFHT_ASY_MAX_NF; p <<= 1) {
+        if (p >= need) {
+            if (best < 0 || p < best) best = p;
+            break;
+        }
+        if (3 * p >= need && (best < 0 || 3 * p < best)) best = 3 * p;
+        if (9 * p >= need && 9 * p > 4096 && (best < 0 || 9 * p < best)) best = 9 * p;
+    }
+    return best;
+}
+
+int nufht_asy_grid(double X, double h, int kw, int *L, int *Nf)
+{
+    /* type-3 NUFFT grid of an asymptotic box (upsampling sigma = 2):
+     * positions l in [-L, L], L = ceil(X / h) + kw / 2 + 2, FFT length
+     * Nf >= sigma (2 L + 1) and >= 2 kw + 8 */
+    const double sigma = 2.0;
+    double Lf = ceil(X / h) + (double)(kw / 2) + 2.0;
+    if (!(Lf >= 0.0) || !(sigma * (2.0 * Lf + 1.0) <= (double)NUFHT_ASY_MAX_NF)) {
+        nufht_set_error("an asymptotic box needs a uniform grid of %.3g points (half-width "
+                        "X = %.6g, spacing h = %.6g): above the supported %lld",
+                        sigma * (2.0 * Lf + 1.0), X, h, (long long)NUFHT_ASY_MAX_NF);
+        return -541;
+    }
+    long long l = (long long)Lf, need = (long long)(sigma * (double)(2 * l + 1));
+    if (need < 2LL * kw + 8) need = 2LL * kw + 8;
+    long long nf = fft_length_for(need);
+    if (nf < 0) {
+        nufht_set_error("no FFT length of at most %lld holds %lld points",
+                        (long long)NUFHT_ASY_MAX_NF, need);
+        return -541;
+    }
+    *L = (int)l;
+    *Nf = (int)nf;
+    return 0;
+}
+
 /* reference fill_plan_parameters, src/fast_hankel_nufht.c:322-414 */
 static int fill_params(NufhtPlan *p, double nu, const NufhtOptions *o)
 {
@@ -235,6 +277,15 @@ static int fill_params(NufhtPlan *p, double nu, const NufhtOptions *o)
     int two_int = is_int(2.0 * nu), nu_int = is_int(nu);
     if (!two_int || (nu_int && fabs(nu) > 200.0) || (!nu_int && fabs(nu) > 9.5)) {
         nufht_set_error("only integer |nu| <= 200 and half-integer |nu| <= 19/2 are supported");
+        return -2;
+    }
+    /* The reference accepts nu = -(n + 1/2) and computes J_{n+1/2} (it takes
+     * |nu| everywhere, src/fast_hankel_nufht.c:336-344, :932).  J_{-(n+1/2)}
+     * is a different function (DLMF 10.47.3 with y_n, singular at 0), so
+     * such orders are refused instead of answered with the wrong one. */
+    if (!nu_int && nu < 0.0) {
+        nufht_set_error("negative half-integer orders are not supported: J_{-(n+1/2)} is "
+                        "singular at 0 and is not J_{n+1/2}");
         return -2;
     }
     int i_idx = (int)llround(fabs(nu));
