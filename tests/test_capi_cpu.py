@@ -164,3 +164,50 @@ def test_pynufht_argument_errors_without_device():
     # as in the reference, the single-column path reports and returns zeros
     out = pn.nufht(0.3, r, np.zeros(20), k)
     assert out.shape == (10,) and np.all(out == 0.0)
+
+
+def test_negative_half_integer_orders_refused():
+    """J_{-(n+1/2)} is not J_{n+1/2} (the reference computes the latter):
+    such orders are refused at plan time with -2."""
+    L = na.lib()
+    r = np.array([1.0, 2.0])
+    for nu in (-0.5, -1.5, -9.5):
+        plan = na.CPlan()
+        rc = L.nufht_plan_init(ctypes.byref(plan), nu, r.ctypes.data_as(na._DP), 2,
+                               r.ctypes.data_as(na._DP), 2, None)
+        assert rc == -2, nu
+        assert "negative half-integer" in na.last_error()
+    plan = na.CPlan()
+    assert L.nufht_plan_init(ctypes.byref(plan), 1.5, r.ctypes.data_as(na._DP), 2,
+                             r.ctypes.data_as(na._DP), 2, None) == 0
+    L.nufht_plan_free(ctypes.byref(plan))
+
+
+def test_asymptotic_grid_size_guard():
+    """Uniform grids of asymptotic boxes: L = ceil(X/h) + kw/2 + 2, FFT length
+    the smallest 2^a, 3 2^a (or 9 2^a above 4096) of at least 2(2L + 1);
+    lengths beyond 2^30 are refused (-541) instead of overflowing."""
+    L = na.lib()
+    L.nufht_asy_grid.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    Lh, Nf = ctypes.c_int(), ctypes.c_int()
+
+    def ok_len(n):
+        for f in (1, 3, 9):
+            q = n // f
+            if n % f == 0 and q & (q - 1) == 0 and (f < 9 or n > 4096):
+                return True
+        return False
+
+    for X, S, kw in [(500.0, 500.0, 10), (0.5, 1e-3, 4), (12.0, 700.0, 12), (1e4, 1e4, 16)]:
+        h = np.pi / (2.0 * S)
+        assert L.nufht_asy_grid(X, h, kw, ctypes.byref(Lh), ctypes.byref(Nf)) == 0
+        assert Lh.value == int(np.ceil(X / h)) + kw // 2 + 2
+        need = max(2 * (2 * Lh.value + 1), 2 * kw + 8)
+        assert Nf.value >= need and ok_len(Nf.value)
+        # the next smaller admissible length would not hold the grid
+        assert not any(ok_len(v) for v in range(need, Nf.value))
+    for X, S in [(1e6, 1e6), (3e5, 1e6), (1e9, 1.0)]:
+        assert L.nufht_asy_grid(X, np.pi / (2.0 * S), 10, ctypes.byref(Lh),
+                                ctypes.byref(Nf)) == -541
+        assert "above the supported" in na.last_error()
