@@ -52,27 +52,47 @@ def say(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def stage_work(st: dict, B: int) -> dict:
-    """Algorithmic work of one launch of each single-kernel stage.
+def stage_work(st: dict) -> dict:
+    """Algorithmic work of one launch of each single-kernel stage, from the
+    plan statistics of the program that ran it.
 
     A channel is one complex series (exponent e, column t); a box has
     C = 2 (K_asy + 1) B of them.  The per-unit figures are the survey's
     cost model: 2 w flops per (source, channel) spread, 4 w flops per
     (target, channel) interpolated (w = kernel width), 2 B flops per direct
-    (target, source) pair with the Bessel evaluation not counted.  FFT: one
-    read and one write of every complex grid point of FP64 (16 B) or FP32
-    (8 B) per pass, two passes counted.
+    (target, source) pair with the Bessel evaluation not counted, 2 (K_loc +
+    1) B flops per local-expansion source (projection) and target
+    (evaluation).  FFT: the bytes the two-pass kernels must move (pass 1
+    reads the 2L + 1 spread positions of a channel and writes its Nf points,
+    pass 2 reads Nf and writes the frequencies interpolation reads).
     """
     kw = st["kernel_width"]
-    cb = 8.0 if st["precision"] == 2 else 16.0
     return {
         "asy_spread": ("flop", 2.0 * kw * st["spread_sc_valu"]),
         "asy_spread_mfma": ("flop", 2.0 * kw * st["spread_sc_mfma"]),
         "asy_interp": ("flop", 4.0 * kw * st["interp_tc_valu"]),
         "asy_interp_mfma": ("flop", 4.0 * kw * st["interp_tc_mfma"]),
-        "direct": ("flop", 2.0 * B * st["pairs_dir"]),
-        "asy_fft": ("byte", 2.0 * 2.0 * cb * st["asy_grid_points"]),
+        "direct": ("flop", st["model_dir"]),
+        "loc_project": ("flop", st["model_loc"] * st["loc_src"] / max(1, st["loc_src"] + st["loc_tgt"])),
+        "loc_eval": ("flop", st["model_loc"] * st["loc_tgt"] / max(1, st["loc_src"] + st["loc_tgt"])),
+        "asy_fft": ("byte", st["fft_bytes"]),
     }
+
+
+def path_model(stats: list, t_measured_ms: float) -> dict:
+    """Whole-path roofline of one step: the survey's algorithmic work of every
+    application the step runs (FP64 and FP32 flops, HBM bytes; the plan
+    statistics' model_* totals), its floor t_roof = max(F64 / 78.6 TFLOP/s +
+    F32 / 157.3 TFLOP/s, bytes / 8 TB/s) and achieved = t_roof / t_measured."""
+    f64 = sum(st["model_f64"] for st in stats)
+    f32 = sum(st["model_f32"] for st in stats)
+    by = sum(st["model_bytes"] for st in stats)
+    t_flop = f64 / (PEAK_FP64_TFLOPS * 1e12) + f32 / (PEAK_FP32_TFLOPS * 1e12)
+    t_byte = by / (PEAK_HBM_GBS * 1e9)
+    t_roof = max(t_flop, t_byte) * 1e3
+    return {"t_roof_ms": t_roof, "t_measured_ms": t_measured_ms,
+            "achieved": t_roof / t_measured_ms, "bound": "flop" if t_flop >= t_byte else "hbm",
+            "fp64_flops": f64, "fp32_flops": f32, "bytes": by, "applications": len(stats)}
 
 
 def _traffic(stage: str, tag: str):
@@ -85,31 +105,68 @@ def _traffic(stage: str, tag: str):
     return float(m.group(1)) if m else None
 
 
-def roofline(prof: dict, works: list, mixed: bool, tag: str) -> dict:
-    """Dominant kernel over the timed region (the single-kernel stages; the
-    FFT stage is several rocFFT kernels and is reported by fft_stage);
-    works = per-plan stage_work dicts of the launches that ran (forward and
-    adjoint plans)."""
-    cand = [s for s in ("direct", "asy_spread", "asy_spread_mfma", "asy_interp",
-                        "asy_interp_mfma")
+def roofline(prof: dict, works: list, mixed: bool, tag) -> dict:
+    """Dominant kernel over a profiled region (the single-kernel stages; the
+    FFT stage is two kernels per size class and is reported by fft_stage);
+    works = stage_work dicts of the programs whose launches ran, each program
+    launched equally often (forward and adjoint)."""
+    cand = [s for s in ("direct", "loc_project", "loc_eval", "asy_spread", "asy_spread_mfma",
+                        "asy_interp", "asy_interp_mfma")
             if prof.get(s, {}).get("launches", 0) > 0]
     dom = max(cand, key=lambda s: prof[s]["ms"])
     launches = prof[dom]["launches"]
     avg_ms = prof[dom]["ms"] / launches
     kind = works[0][dom][0]
-    per_launch = sum(w[dom][1] for w in works) / len(works)
+    # launches of this stage per pass over the programs (a program without
+    # work for the stage does not launch it)
+    ran = [w[dom][1] for w in works if w[dom][1] > 0]
+    per_launch = sum(ran) / max(1, len(ran))
     if kind == "byte":
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     else:
-        peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
+        peak = PEAK_FP32_TFLOPS if (mixed and dom.startswith("asy")) else PEAK_FP64_TFLOPS
         achieved = per_launch / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s"}
+        roof = {"bound": "mfma" if "mfma" in dom or dom == "loc_project" else "valu",
+                "achieved": achieved, "peak": peak, "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = _traffic(dom, tag)
+    roof["traffic"] = _traffic(dom, tag) if tag else None
     roof.update({"kernel": dom, "avg_launch_ms": avg_ms, "launches": launches,
                  "algorithmic_per_launch": per_launch})
     return roof
+
+
+def fft_stage(prof: dict, works: list):
+    """The uniform-FFT stage against HBM: bytes the kernels must move (pruned)
+    per stage launch over its measured time."""
+    if prof.get("asy_fft", {}).get("launches", 0) == 0:
+        return None
+    f_ms = prof["asy_fft"]["ms"] / prof["asy_fft"]["launches"]
+    ran = [w["asy_fft"][1] for w in works if w["asy_fft"][1] > 0]
+    f_b = sum(ran) / max(1, len(ran))
+    return {"bound": "hbm", "avg_stage_ms": f_ms, "algorithmic_bytes": f_b,
+            "achieved_GBps": f_b / (f_ms * 1e-3) / 1e9,
+            "frac": f_b / (f_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+
+
+def _profiled(fn, steps: int) -> dict:
+    """Per-stage device times over `steps` calls of fn (HIP events around
+    every stage launch, on the launch stream)."""
+    import torch
+    import nufht_amd as na
+    torch.cuda.synchronize()
+    na.profile(True)
+    na.profile_reset()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    prof = na.profile_read()
+    na.profile(False)
+    return prof
+
+
+def _stages(prof: dict, steps: int) -> dict:
+    return {s: round(v["ms"] / steps, 4) for s, v in prof.items() if v["launches"] > 0}
 
 
 def _dev_cols(a: np.ndarray, dev):
@@ -165,15 +222,12 @@ def run_headline(args, dev):
     mixed = st_f["precision"] == 2
     prec = "mixed" if mixed else "fp64"
     tag = f"nu5_M1e6_N1e6_B{B}_tol1e-8_{prec}"
-    works = [stage_work(st_f, B), stage_work(st_a, B)]
+    works = [stage_work(st_f), stage_work(st_a)]
     roof = roofline(prof, works, mixed, tag)
-    fft = None
-    if prof.get("asy_fft", {}).get("launches", 0) > 0:
-        f_ms = prof["asy_fft"]["ms"] / prof["asy_fft"]["launches"]
-        f_b = sum(w["asy_fft"][1] for w in works) / len(works)
-        fft = {"bound": "hbm", "avg_stage_ms": f_ms, "algorithmic_bytes": f_b,
-               "achieved_GBps": f_b / (f_ms * 1e-3) / 1e9,
-               "frac": f_b / (f_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+    path = path_model([st_f, st_a], ms_step)
+    roof["achieved_path"] = path["achieved"]
+    roof["path"] = path
+    fft = fft_stage(prof, works)
 
     plain = None
     if args.plain_steps > 0:
@@ -192,7 +246,7 @@ def run_headline(args, dev):
         plain = {"value": pv, "ms_per_step": (p1 - p0) * 1e3 / args.plain_steps,
                  "speedup_batched_over_plain": value / pv}
     plan.close()
-    stages = {s: round(v["ms"] / args.steps, 4) for s, v in prof.items() if v["launches"] > 0}
+    stages = _stages(prof, args.steps)
     return {"value": value, "ms_per_step": ms_step, "M": M, "N": N, "B": B, "nu": nu,
             "tol": tol, "plan_s": plan_s, "first_call_s": first_s, "stages_ms": stages,
             "roofline": roof, "fft_stage": fft, "plain": plain, "precision": prec}
@@ -242,6 +296,17 @@ def run_uneven(dev, tol: float, steps: int):
         dt = (time.perf_counter() - t0) / steps
         out[name] = {"value": colpts / dt, "ms_per_step": dt * 1e3}
     out["speedup_batched_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
+    # per-stage times from a separate profiled pass (the launches run eagerly
+    # there: events cannot sit inside the replayed graph)
+    st_f, st_a = mf.stats(), ma.stats()
+    prof = _profiled(lambda: (mf.execute(cs, gs, use_graph=False),
+                              ma.execute(gs, xs, use_graph=False)), steps)
+    works = [stage_work(st_f), stage_work(st_a)]
+    out["roofline"] = roofline(prof, works, False, None)
+    out["roofline"]["path"] = path_model([st_f, st_a], out["batched"]["ms_per_step"])
+    out["roofline"]["achieved_path"] = out["roofline"]["path"]["achieved"]
+    out["fft_stage"] = fft_stage(prof, works)
+    out["stages_ms_per_step_eager"] = _stages(prof, steps)
     mf.close()
     ma.close()
     for p in pf + pa:
@@ -275,6 +340,18 @@ def run_mixed(dev, steps: int):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
         out[prec] = {"value": colpts / dt, "ms_per_step": dt * 1e3}
+        st_f = plan.stats(B)
+        adj = na.Plan(1.0, k, r, tol=1e-6, precision=prec)
+        st_a = adj.stats(B)
+        adj.close()
+        prof = _profiled(lambda: plan.adjoint(plan.forward(c)), steps)
+        works = [stage_work(st_f), stage_work(st_a)]
+        roof = roofline(prof, works, prec == "mixed", None)
+        roof["path"] = path_model([st_f, st_a], dt * 1e3)
+        roof["achieved_path"] = roof["path"]["achieved"]
+        out[prec]["roofline"] = roof
+        out[prec]["fft_stage"] = fft_stage(prof, works)
+        out[prec]["stages_ms_per_step"] = _stages(prof, steps)
         plan.close()
     gf, xf = res["fp64"]
     gm, xm = res["mixed"]
@@ -289,6 +366,7 @@ def run_gradient(dev, steps: int):
     ||H(c; r, k) - y||^2 through the autograd layer, forward + backward
     (dL/dc by the adjoint, dL/dr and dL/dk by the node-gradient transforms)."""
     import torch
+    import nufht_amd as na
     from nufht_amd import workloads as W
     from nufht_amd.autograd import NufhtOperator
     wl = W.gradient()
@@ -315,29 +393,93 @@ def run_gradient(dev, steps: int):
         step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    prof = _profiled(step, steps)
     op.close()
+    # the step's applications: forward H, backward H^T (dL/dc), and for the
+    # node gradients H^T and H^T of order nu + 1 (dL/dr), H and H of order
+    # nu + 1 (dL/dk)
+    r, k, B = wl["r"], wl["k"], 8
+    sts = {}
+    for name, (nu_, a, b) in {"fwd": (1.0, r, k), "adj": (1.0, k, r), "fwd_up": (2.0, r, k),
+                              "adj_up": (2.0, k, r)}.items():
+        p = na.Plan(nu_, a, b, tol=1e-10, precision="fp64")
+        sts[name] = p.stats(B)
+        p.close()
+    apps = [sts["fwd"], sts["fwd"], sts["adj"], sts["adj"], sts["fwd_up"], sts["adj_up"]]
+    works = [stage_work(st) for st in apps]
+    roof = roofline(prof, works, False, None)
+    roof["path"] = path_model(apps, dt * 1e3)
+    roof["achieved_path"] = roof["path"]["achieved"]
     return {"M": wl["r"].size, "N": wl["k"].size, "columns": 8, "tol": 1e-10,
-            "ms_forward_backward": dt * 1e3}
+            "ms_forward_backward": dt * 1e3, "roofline": roof,
+            "stages_ms_per_step": _stages(prof, steps)}
 
 
 # ---------------------------------------------------------------- CPU baseline
-def run_cpu_baseline(sub: int = 8):
-    """The oracle's CPU restatement of the reference algorithm (1 thread, as
-    the reference runs its NUFFTs) on every sub-th node of the headline
-    grids, one column, forward + adjoint."""
+def _cpu_info() -> dict:
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_available": avail}
+
+
+def _time_port(nu, r, c, k, tol, threads):
+    """Forward + adjoint through the oracle's CPU restatement of the
+    reference's nufht_batch (boxes, Wimp and Hankel expansions, its own CPU
+    type-3 NUFFT); column groups split over `threads` OpenMP threads."""
     import oracle
-    from nufht_amd import workloads as W
-    wl = W.large_uniform(batch=1)
-    r, k, c = wl["r"][::sub], wl["k"][::sub], wl["cs"][::sub, :1]
-    say(f"cpu baseline: M=N={r.size} 1 column, 1 thread")
     t0 = time.perf_counter()
-    g = oracle.nufht_batch(wl["nu"], r, c, k, tol=wl["tol"], asy_mode=1, nthreads=1)
-    oracle.nufht_batch(wl["nu"], k, g, r, tol=wl["tol"], asy_mode=1, nthreads=1)
-    dt = time.perf_counter() - t0
-    colpts = 2.0 * (r.size + k.size)
-    return {"value": colpts / dt, "unit": "column-points/s", "cores": 1, "kind": "port",
+    g = oracle.nufht_batch(nu, r, c, k, tol=tol, asy_mode=1, nthreads=threads)
+    oracle.nufht_batch(nu, k, g, r, tol=tol, asy_mode=1, nthreads=threads)
+    return time.perf_counter() - t0
+
+
+def run_cpu_baseline(sub: int = 8):
+    """The oracle's CPU restatement of the reference algorithm on the GPU
+    box's host: the headline workload on every sub-th node (M = N = 125000)
+    with all 16 columns over all host threads available to this process (at
+    most 16, one GPU's share; one column group per thread), and the same
+    sample with one column on one thread (the reference's own threading:
+    FINUFFT nthreads = 1, no OpenMP regions); plus the Gauss-Legendre nu = 0
+    workload (10000 nodes, 200 k, 4 columns) in full at both."""
+    from nufht_amd import workloads as W
+    info = _cpu_info()
+    threads = max(1, min(16, info["cpus_available"]))
+    wl = W.large_uniform()
+    r, k, cs = wl["r"][::sub], wl["k"][::sub], wl["cs"][::sub]
+    B = cs.shape[1]
+    say(f"cpu baseline: M=N={r.size} {B} columns, {threads} threads")
+    dt_all = _time_port(wl["nu"], r, cs, k, wl["tol"], threads)
+    say(f"cpu baseline: M=N={r.size} 1 column, 1 thread")
+    dt_one = _time_port(wl["nu"], r, cs[:, :1], k, wl["tol"], 1)
+    gl = W.gaussian_gl()
+    dt_gl1 = _time_port(0.0, gl["r"], gl["cs"], gl["k"], 1e-8, 1)
+    dt_glt = _time_port(0.0, gl["r"], gl["cs"], gl["k"], 1e-8, min(threads, 4))
+    cp = 2.0 * (r.size + k.size)
+    cp_gl = 2.0 * gl["cs"].shape[1] * (gl["r"].size + gl["k"].size)
+    return {"value": B * cp / dt_all, "unit": "column-points/s", "cores": threads,
+            "kind": "port",
             "sample": f"nu=5 tol=1e-8, every {sub}th node of the M=N=1e6 U[0,1000) grids "
-                      f"(M=N={r.size}), 1 column, forward+adjoint, {dt:.1f} s"}
+                      f"(M=N={r.size}), {B} columns, forward+adjoint, {threads} threads "
+                      f"(one column group each), {dt_all:.1f} s",
+            **info,
+            "one_thread": {"value": cp / dt_one, "cores": 1,
+                           "sample": f"the same nodes, 1 column, forward+adjoint, {dt_one:.1f} s"},
+            "gauss_legendre_full": {
+                "workload": "nu=0 tol=1e-8, GL10000 on [0,50], k=10(j+1)/200, 4 columns, "
+                            "forward+adjoint",
+                "one_thread": cp_gl / dt_gl1, "threads": min(threads, 4),
+                "all_threads": cp_gl / dt_glt, "seconds": [dt_gl1, dt_glt]}}
 
 
 def copy_calibration(dev):
