@@ -40,6 +40,27 @@ int nufht_batch_device(const NufhtPlan *plan,
                        double *d_gs, int ld_gs,
                        int batch, void *stream);
 
+/* ---- allocation-free execution ------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory during a call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the transform of nufht_batch_device,
+ * calling no hipMalloc, hipFree or allocating library routine.  The kernels'
+ * scratch (local-expansion moments; the sorted-order copies of cs and gs
+ * when the plan's grids were not ascending) lives in a static device array
+ * of 2^24 doubles (128 MiB) that each GPU the library runs on holds until
+ * the process ends.  The plan's launch program must exist already: run
+ * nufht_batch_device once with the same batch (and the plan's current
+ * precision and knobs) first; that call allocates the plan's own state.
+ * All calls on one GPU share that GPU's scratch array: calls that overlap
+ * on one GPU (other streams, other threads, other plans) race; order them.
+ * Returns 0, -400/-401 (as nufht_batch_device), -550 (the scratch this call
+ * needs, moments plus m*batch and/or n*batch staged values, exceeds 2^24
+ * doubles), -551 (no launch program for this batch yet), or -5xx. */
+int nufht_batch_device_noalloc(const NufhtPlan *plan,
+                               const double *d_cs, int ld_cs,
+                               double *d_gs, int ld_gs,
+                               int batch, void *stream);
+
 /* ---- adjoint ---------------------------------------------------------------
  * x[j,t] = sum_i y[i,t] J_nu(ws_i rs_j): the transposed transform, i.e. the
  * reference's nufht_batch(nu, ws, y, rs) with sources and targets swapped. */

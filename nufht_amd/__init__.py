@@ -111,7 +111,7 @@ EXPORTS = [
     "nufht_amd_plan_stats", "nufht_amd_autotune", "nufht_amd_version",
     "nufht_amd_last_error", "nufht_amd_get_knobs", "nufht_amd_set_knobs",
     "nufht_multi_autotune", "nufht_multi_get_knobs", "nufht_amd_plan_cache",
-    "nufht_amd_write_table_cache", "nufht_multi_stats",
+    "nufht_amd_write_table_cache", "nufht_batch_device_noalloc", "nufht_multi_stats",
 ]
 
 
@@ -148,6 +148,7 @@ def lib():
     L.nufht_amd_plan_set_precision.argtypes = [PP, i]
     L.nufht_amd_plan_precision.argtypes = [PP]
     L.nufht_batch_device.argtypes = [PP, vp, i, vp, i, i, vp]
+    L.nufht_batch_device_noalloc.argtypes = [PP, vp, i, vp, i, i, vp]
     L.nufht_adjoint_device.argtypes = [PP, vp, i, vp, i, i, vp]
     L.nufht_adjoint.argtypes = [PP, _DP, i, _DP, i, i]
     L.nufht_grad_nodes_device.argtypes = [PP, vp, i, vp, i, i, vp, vp, vp]

@@ -1390,6 +1390,95 @@ extern "C" int nufht_batch_device(const NufhtPlan *plan, const double *d_cs, int
     return rc;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free execution (nufht_batch_device_noalloc).  The kernels'
+// scratch, i.e. the local-expansion moments (partial sums, cleared by a fill
+// kernel at the start of every call) and the sorted-order staging of
+// callers whose grids are not ascending, lives in this file's static device
+// array instead of buffers the library allocates.  Each GPU the library
+// runs on holds its own copy of the array until the process ends; host code
+// reaches it only through the per-device address from hipGetSymbolAddress.
+constexpr long long kScratchDoubles = 1LL << 24;   // 128 MiB per GPU
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kScratchMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kScratchMaxDevices];
+
+// Device address of the current device's scratch array (looked up once per
+// device, with that device current).
+static int scratch_ptr(double **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchMaxDevices) {
+        nufht_set_error("device %d is beyond the %d the scratch table covers", dev,
+                        kScratchMaxDevices);
+        return -550;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+extern "C" int nufht_batch_device_noalloc(const NufhtPlan *plan, const double *d_cs, int ld_cs,
+                                          double *d_gs, int ld_gs, int batch, void *stream)
+{
+    if (!plan || !plan->impl || !d_cs || !d_gs || batch <= 0) return -400;
+    if (ld_cs < plan->m || ld_gs < plan->n) return -401;
+    NufhtImplHost *h = (NufhtImplHost *)plan->impl;
+    const int m = plan->m, n = plan->n, B = batch;
+    const long long n_in = h->perm_r ? (long long)m * B : 0, n_out = h->perm_w ? (long long)n * B : 0;
+    if (n_in + n_out > kScratchDoubles) {
+        nufht_set_error("nufht_batch_device_noalloc: staging of %lld doubles exceeds the scratch "
+                        "array's %lld (m, n = %d, %d unsorted nodes, batch %d)",
+                        n_in + n_out, kScratchDoubles, m, n, B);
+        return -550;
+    }
+    EngineState *E = (EngineState *)h->dev;
+    if (!E) {
+        nufht_set_error("nufht_batch_device_noalloc: the plan has no device state yet; run "
+                        "nufht_batch_device once with batch %d first", B);
+        return -551;
+    }
+    std::lock_guard<std::mutex> g(E->mu);
+    if (!E->dev_ready || !E->prog || E->prog_B != B || E->prog_prec != effective_prec(plan) ||
+        E->prog_knobs != E->knobs) {
+        nufht_set_error("nufht_batch_device_noalloc: the plan has no launch program for batch %d "
+                        "with its current precision and knobs; run nufht_batch_device once with "
+                        "this batch first", B);
+        return -551;
+    }
+    Program *P = E->prog;
+    if (P->ncoef + n_in + n_out > kScratchDoubles) {
+        nufht_set_error("nufht_batch_device_noalloc: %lld local-expansion moments and %lld "
+                        "staged values exceed the scratch array's %lld doubles",
+                        P->ncoef, n_in + n_out, kScratchDoubles);
+        return -550;
+    }
+    double *base;
+    RC_TRY(scratch_ptr(&base));
+    double *coef = base, *s_in = base + P->ncoef, *s_out = s_in + n_in;
+    hipStream_t st = (hipStream_t)stream;
+    RC_TRY(order_after_last(E, st));
+    const double *cs_use = d_cs;
+    long long ld_in = ld_cs;
+    if (h->perm_r) {
+        RC_TRY(gather_rows(d_cs, ld_cs, s_in, m, E->dev.d_perm_r, m, B, st));
+        cs_use = s_in;
+        ld_in = m;
+    }
+    set_io(P, 0, cs_use, ld_in, h->perm_w ? s_out : d_gs, h->perm_w ? n : ld_gs);
+    RC_TRY(run_program(P, st, coef));
+    if (h->perm_w) RC_TRY(scatter_rows(s_out, n, d_gs, ld_gs, E->dev.d_perm_w, n, B, st));
+    RC_TRY(mark_done(E, st));
+    return 0;
+}
+
 extern "C" int nufht_engine_batch_host(const NufhtPlan *plan, const double *cs, int ld_cs, double *gs,
                                        int ld_gs, int batch, NufhtScratch *scratch)
 {
