@@ -164,7 +164,9 @@ int nufht_multi_stats(const NufhtMulti *mt, NufhtPlanStats *st);
  *                 kernel widths <= 8, else 1024)
  *   spread_split  sources per VALU spread item
  *   interp_mfma   1: 16-target tiles whose kernel supports fit a 16-position
- *                 window interpolate on the matrix cores; 0: VALU only
+ *                 window interpolate on the matrix cores, in runs of up to 8
+ *                 tiles whose window is staged in LDS once; 2: the same tiles
+ *                 one per item, windows read from HBM; 0: VALU only
  *   own_fft       1: two-pass FFT kernels for the lengths they cover; 0: rocFFT
  *   spread_sched  matrix-core spread schedule: 1 split (4 waves stage source
  *                 chunks into double-buffered LDS while 4 multiply), 0 unified

@@ -65,11 +65,16 @@ void asy_interp(Program *P, hipStream_t st)
     }
     if (!P->aimf.empty()) {
         StageTimer tm(ST_INTERP_MF, st);
-        // FP64 products for both grid precisions: the FP32 matrix-core form of
-        // this kernel measured 1.7x slower on gfx950
-        hipLaunchKernelGGL((k_asy_interp_mfma<R, double, EM>), dim3((unsigned)P->aimf.size()),
-                           dim3(64), 0, st, P->d_tfs, P->d_abox, P->d_aimf,
-                           (const G *)P->d_grid2, P->d_ft);
+        // the products in the grid's precision: FP32 matrix cores (twice the
+        // FP64 rate) for the FP32 grids of mixed precision
+        hipLaunchKernelGGL((k_asy_interp_mfma<R, R, EM>), dim3((unsigned)P->aimf.size()), dim3(256),
+                           0, st, P->d_tfs, P->d_abox, P->d_aimf, (const G *)P->d_grid2, P->d_ft);
+    }
+    if (!P->aimt.empty()) {
+        StageTimer tm(ST_INTERP_MF, st);
+        hipLaunchKernelGGL((k_asy_interp_tile<R, double, EM>), dim3((unsigned)P->aimt.size()),
+                           dim3(64), 0, st, P->d_tfs, P->d_abox, P->d_aimt, (const G *)P->d_grid2,
+                           P->d_ft);
     }
 }
 

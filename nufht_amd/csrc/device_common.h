@@ -57,6 +57,11 @@ struct AsyBoxD {
     double C, D, h, delta, a1, a2, rho;
 };
 struct AsyItem { int box, start, count, col0, tb, pad; };
+// targets [start, start + count) of an asymptotic box, interpolated from the
+// grid positions [mw0, mw0 + npos)
+struct InterpRun { int box, start, count, mw0, npos, pad; };
+constexpr int kIrTiles = 8;    // 16-target tiles per matrix-core interpolation run
+constexpr int kIrPos = 64;     // grid positions a run stages, at most
 struct ZeroItem { int tf, col, start, count; };
 
 constexpr int kNFT = 40;   // Chebyshev terms of the kernel transform table

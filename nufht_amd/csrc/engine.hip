@@ -13,6 +13,7 @@
 #include <rocfft/rocfft.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -59,6 +60,20 @@ extern "C" const char *nufht_amd_version(void) { return "nufht-amd 0.1 gfx950"; 
         int rc_ = (expr);              \
         if (rc_ != 0) return rc_;      \
     } while (0)
+
+// Phase times of program builds on stderr when NUFHT_AMD_TRACE is set.
+static const bool g_trace = getenv("NUFHT_AMD_TRACE") != nullptr;
+struct PhaseClock {
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(const char *what)
+    {
+        if (!g_trace) return;
+        auto n = std::chrono::steady_clock::now();
+        fprintf(stderr, "[nufht_amd] %-24s %8.2f ms\n", what,
+                std::chrono::duration<double, std::milli>(n - t).count());
+        t = n;
+    }
+};
 
 // ---------------------------------------------------------------------------
 // per-stage timing
@@ -399,6 +414,7 @@ static void free_program(Program *P)
     hipFree(P->d_grid2); hipFree(P->d_fftw); hipFree(P->d_fft_chinfo);
     hipFree(P->d_azero); hipFree(P->d_arange); hipFree(P->d_adense); hipFree(P->d_adrange);
     hipFree(P->d_aimf);
+    hipFree(P->d_aimt);
     hipFree(P->d_cheb);
     hipFree(P->d_jtab);
     for (auto &c : P->fft) {
@@ -489,6 +505,7 @@ static EngineState *engine_of(const NufhtPlan *plan)
 static int ensure_dev(const NufhtPlan *plan, EngineState *E)
 {
     if (E->dev_ready) return 0;
+    PhaseClock clk;
     NufhtImplHost *h = (NufhtImplHost *)plan->impl;
     HIP_TRY(hipMalloc((void **)&E->dev.d_r, sizeof(double) * plan->m));
     HIP_TRY(hipMalloc((void **)&E->dev.d_w, sizeof(double) * plan->n));
@@ -503,6 +520,7 @@ static int ensure_dev(const NufhtPlan *plan, EngineState *E)
         HIP_TRY(hipMemcpy(E->dev.d_perm_w, h->perm_w, sizeof(int) * plan->n, hipMemcpyHostToDevice));
     }
     E->dev_ready = true;
+    clk.mark("nodes to device");
     return 0;
 }
 
@@ -692,6 +710,75 @@ static void add_dir_items(Program *P, int tf, int B, const std::vector<Box> &dir
     }
 }
 
+// Interpolation items, per asymptotic box in order.  The box's targets are
+// cut into 16-target tiles; a tile whose kernel supports fit a 16-position
+// window (one position of slack; a 24-position form measured slower than
+// the VALU kernel on the tiles it would add) goes to the matrix-core kernel,
+// in runs of up to kIrTiles consecutive such tiles whose windows lie within
+// kIrPos positions of the run's first window start; the other tiles form
+// VALU items of up to 256/tb targets and tb columns.
+static void add_interp_items(Program *P, const std::vector<TfInput> &in)
+{
+    for (size_t bi = 0; bi < P->abox.size(); ++bi) {
+        const int box = (int)bi;
+        const AsyBoxD &A = P->abox[bi];
+        const TfDesc &T = P->tfs[A.tf];
+        const NufhtImplHost *h = (const NufhtImplHost *)in[A.tf].plan->impl;
+        int tb = 1;
+        while (tb < T.B && tb < 16) tb <<= 1;
+        const int tp = 256 / tb;
+        auto m0_of = [&](int s) { return (int)std::ceil((h->w[A.i0 + s] - A.D - A.a2) / A.delta); };
+        int run0 = -1;   // start of the pending VALU run of tiles
+        auto flush_valu = [&](int upto) {
+            for (int c0 = 0; run0 >= 0 && c0 < T.B; c0 += tb)
+                for (int s = run0; s < upto; s += tp) {
+                    const int n = std::min(tp, upto - s);
+                    P->ainterp.push_back({box, s, n, c0, tb, 0});
+                    P->stats.interp_tc_valu += (long long)n * A.E * std::min(tb, T.B - c0);
+                }
+            run0 = -1;
+        };
+        InterpRun cur = {box, 0, 0, 0, 0, 0};
+        auto flush_run = [&]() {
+            if (cur.count) {
+                P->aimf.push_back(cur);
+                P->stats.interp_tc_mfma += (long long)cur.count * A.E * T.B;
+            }
+            cur.count = 0;
+        };
+        const bool tiles = P->knobs.interp_mfma == 2;
+        for (int s = 0; s < A.ni; s += 16) {
+            const int cnt = std::min(16, A.ni - s);
+            const int m0f = m0_of(s), m0l = m0_of(s + cnt - 1);
+            if (!(P->knobs.interp_mfma && m0l - m0f + T.kw + 2 <= 16)) {
+                flush_run();
+                if (run0 < 0) run0 = s;
+                continue;
+            }
+            flush_valu(s);
+            const int mw = m0f - 1;    // the tile's window: [mw, mw + 16)
+            if (tiles) {
+                // one item per tile, every column chunk
+                P->aimt.push_back({box, s, cnt, 0, tb, mw});
+                P->stats.interp_tc_mfma += (long long)cnt * A.E * T.B;
+                continue;
+            }
+            if (cur.count && (cur.count == 16 * kIrTiles || mw + 16 - cur.mw0 > kIrPos ||
+                              mw < cur.mw0))
+                flush_run();
+            if (!cur.count) {
+                cur.start = s;
+                cur.mw0 = mw;
+                cur.npos = 16;
+            }
+            cur.count += cnt;
+            cur.npos = std::max(cur.npos, mw + 16 - cur.mw0);
+        }
+        flush_run();
+        flush_valu(A.ni);
+    }
+}
+
 // Local expansions the local kernels run: K_loc + 1 terms in at most
 // kLocMaxNlt row tiles of 16, and a per-target scratch (loc_slots words) that
 // 64 targets per block keep within 64 KiB of LDS.
@@ -703,8 +790,10 @@ static bool loc_fits(int order, int half, int K)
 
 static int build_program(Program *P, const std::vector<TfInput> &in)
 {
+    PhaseClock clk;
     static std::once_flag fft_once;
     std::call_once(fft_once, []() { rocfft_setup(); });
+    clk.mark("rocfft setup");
     P->prec = 0;
     for (auto &x : in) P->prec = std::max(P->prec, effective_prec(x.plan) == 2 ? 2 : 0);
     // a program mixes transforms only at one grid precision: FP64 wins
@@ -923,7 +1012,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         const TfDesc &T = P->tfs[A.tf];
         const NufhtImplHost *h = (const NufhtImplHost *)in[A.tf].plan->impl;
         P->abox.push_back(A);
-        int tb = ap.tb, tp = 256 / tb;                 // interpolation targets per item
+        int tb = ap.tb;
         int sp = 256 / std::max(1, tb / 2);            // spread positions per item
         const double *rb = h->r + A.j0, *re = h->r + A.j0 + A.nj;
         // source-dense boxes (many sources per grid cell) use the split-K kernel
@@ -1006,42 +1095,11 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                     }
                 }
             }
-            // interpolation: 16-target tiles whose kernel supports fit a 16-position
-            // window go to the MFMA kernel (pad = window start); the rest are
-            // merged into VALU items of up to 256/tb targets
-            int run0 = -1;
-            auto flush = [&](int upto) {
-                for (int s = run0; run0 >= 0 && s < upto; s += tp) {
-                    P->ainterp.push_back({box, s, std::min(tp, upto - s), c0, tb, 0});
-                    P->stats.interp_tc_valu += (long long)std::min(tp, upto - s) * chans;
-                }
-                run0 = -1;
-            };
-            for (int s = 0; s < A.ni; s += 16) {
-                int cnt = std::min(16, A.ni - s);
-                double sf = h->w[A.i0 + s] - A.D, sl = h->w[A.i0 + s + cnt - 1] - A.D;
-                int m0f = (int)std::ceil((sf - A.a2) / A.delta);
-                int m0l = (int)std::ceil((sl - A.a2) / A.delta);
-                // window of the tile's supports (one position of slack) within
-                // 16 positions: the MFMA kernel (a 24-position form measured
-                // slower than the VALU kernel on the tiles it would add)
-                const int win = m0l - m0f + T.kw + 2;
-                if (P->knobs.interp_mfma && win <= 16) {
-                    flush(s);
-                    // one item per tile covers every column chunk
-                    if (c0 == 0) {
-                        P->aimf.push_back({box, s, cnt, 0, tb, m0f - 1});
-                        P->stats.interp_tc_mfma += (long long)cnt * A.E * T.B;
-                    }
-                } else if (run0 < 0) {
-                    run0 = s;
-                }
-            }
-            flush(A.ni);
         }
         P->stats.asy_grid += A.Nf;
         P->stats.asy_grid_points += (long long)A.E * T.B * A.Nf;
     }
+    add_interp_items(P, in);
     P->stats.fft_classes = (long long)P->fft.size();
     P->emax = e_max <= 6 ? 6 : e_max <= 10 ? 10 : e_max <= 14 ? 14 : 22;
     // split (producer / consumer) schedule of the matrix-core spread unless
@@ -1092,6 +1150,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         P->stats.z_split = ((NufhtImplHost *)in[0].plan->impl)->exec_z;
     }
     P->stats.precision = P->prec;
+    clk.mark("work lists (host)");
     // uploads
     RC_TRY(upload(&P->d_zero, P->zero));
     RC_TRY(upload(&P->d_dir, P->dir));
@@ -1108,6 +1167,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     RC_TRY(upload(&P->d_arange, P->arange));
     RC_TRY(upload(&P->d_adense, P->adense));
     RC_TRY(upload(&P->d_aimf, P->aimf));
+    RC_TRY(upload(&P->d_aimt, P->aimt));
     RC_TRY(upload(&P->d_adrange, P->adrange));
     for (auto &it : P->ainterp) P->interp_tmax = std::max(P->interp_tmax, 256 / it.tb);
     RC_TRY(upload(&P->d_ft, ft_all));
@@ -1120,12 +1180,14 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
             hipLaunchKernelGGL(k_jtab_build, dim3(T.jtab_n), dim3(kJDeg + 1), 0, nullptr, T.order,
                                T.half, T.jtab_n, P->d_jtab + T.jtab_off);
     HIP_TRY(hipDeviceSynchronize());
+    clk.mark("uploads + J tables");
     HIP_TRY(hipMalloc((void **)&P->d_tfs, std::max<size_t>(1, P->tfs.size()) * sizeof(TfDesc)));
     size_t esz = mixed ? sizeof(float2) : sizeof(double2);
     if (P->grid_elems) {
         HIP_TRY(hipMalloc(&P->d_grid, P->grid_elems * esz));
         HIP_TRY(hipMalloc(&P->d_grid2, P->grid_elems * esz));
     }
+    clk.mark("grid allocation");
     // two-pass FFT classes: twiddle tables (in the grid's precision)
     std::vector<double> tw;
     for (auto &c : P->fft) {
@@ -1190,6 +1252,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
             rocfft_execution_info_set_work_buffer(c.info, c.work, c.work_bytes);
         }
     }
+    clk.mark("FFT tables + rocFFT plans");
     return 0;
 }
 
@@ -1893,7 +1956,7 @@ static int knobs_check(const NufhtAmdKnobs *k)
 {
     if (!(k->dense_cell > 0.0) || (k->dense_split != 0 && k->dense_split < kSQ) ||
         k->spread_split < 64 ||
-        (k->interp_mfma != 0 && k->interp_mfma != 1) || (k->own_fft != 0 && k->own_fft != 1) ||
+        k->interp_mfma < 0 || k->interp_mfma > 2 || (k->own_fft != 0 && k->own_fft != 1) ||
         k->spread_sched < -1 || k->spread_sched > 1 ||
         (k->precision != NUFHT_PREC_FP64 && k->precision != NUFHT_PREC_AUTO &&
          k->precision != NUFHT_PREC_MIXED)) {
@@ -1974,7 +2037,7 @@ static int tune_knobs(Knobs &cur, Apply apply, Run run, hipStream_t st)
         return 0;
     };
     for (double v : kTuneDense) { Knobs c = cur; c.dense_cell = v; consider(c); }
-    for (int v = 1; v >= 0; --v) { Knobs c = cur; c.interp_mfma = v; consider(c); }
+    for (int v : {1, 2, 0}) { Knobs c = cur; c.interp_mfma = v; consider(c); }
     for (int v : kTuneDenseSplit) { Knobs c = cur; c.dense_split = v; consider(c); }
     for (int v : kTuneSpreadSplit) { Knobs c = cur; c.spread_split = v; consider(c); }
     for (int v = 1; v >= 0; --v) { Knobs c = cur; c.own_fft = v; consider(c); }

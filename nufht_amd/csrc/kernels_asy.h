@@ -577,7 +577,228 @@ __global__ __launch_bounds__(SpreadSched<SP>::threads, (SP && EMAX <= 10) ? 4 : 
 }
 
 
-// Interpolation, target-dense tiles: 16 consecutive targets whose kernel
+// Interpolation, target-dense runs, on the matrix cores.  An item is a run
+// of up to kIrTiles consecutive 16-target tiles of one box whose kernel
+// supports each fit a window of 16 grid positions, and whose windows all
+// lie in [mw0, mw0 + npos), npos <= kIrPos.  Targets are dense against the
+// grid (consecutive tiles mostly share their window: the headline and
+// clustered workloads read each grid position 10 to 13 times as 16-target
+// windows), so the run's positions are staged in LDS once per (column
+// chunk, exponent group) with coalesced loads of whole channel rows, and
+// every tile's products read their window from there.  For exponent group
+// g (nsub = 16/tb exponents) and tile u:
+//   D[i][(t, e2)] = sum_m phi(s_i - m Delta) grid[e][t][m],  e = g nsub + e2
+// as 16x16x4 FP64 matrix products (4 K steps of 4 positions, the phi
+// operand computed once per tile); the FP64 epilogue applies
+// e^{i(phi0 + w C)} h Delta / psihat(s) and the Hankel weights.  The
+// Hankel combination is folded into a running complex sum per (target,
+// column): S = sum_e A'_e i^(e mod 2) G_e with A'_e = +-a_e (rho w)^-e, and
+// the output is b0 Re(f S) with f = e^{i(phi0 + w C)} h Delta / psihat(s):
+// the real part for even e and minus the imaginary part for odd e, as the
+// reference's cos / sin terms (src/expansions.c:523-531, :549-557).
+constexpr int kIrTargets = 16 * kIrTiles;
+constexpr int kIrStride = kIrPos + 1;        // odd row stride: conflict-free reads
+
+template <typename GR, typename CR, int EMAX>
+__global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restrict__ tfs,
+                                                         const AsyBoxD *__restrict__ boxes,
+                                                         const InterpRun *__restrict__ items,
+                                                         const typename Cplx<GR>::T *__restrict__ grid,
+                                                         const double *__restrict__ ft)
+{
+    typedef CR R;
+    typedef typename Cplx<GR>::T C2;
+    typedef typename Mfma<R>::V V;
+    constexpr int TPW = kIrTiles / 4;
+    __shared__ C2 s_g[16 * kIrStride];           // staged window: [channel][position]
+    __shared__ double s_s[kIrTargets], s_fr[kIrTargets], s_fi[kIrTargets], s_b0[kIrTargets];
+    __shared__ int s_m0[kIrTargets];
+    __shared__ double s_P[kIrTargets][EMAX];
+    __shared__ double s_out[4][16][17];           // per wave: result tile, transposed
+    const InterpRun it = items[xcd_item<4>()];
+    const AsyBoxD bx = boxes[it.box];
+    const TfDesc &T = tfs[bx.tf];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int E = bx.E, kw = T.kw;
+    // per-target factors, one thread per target (rows past the run's end
+    // repeat its last target with zero weight)
+    for (int ti = tid; ti < kIrTargets; ti += 256) {
+        const double w = T.w[bx.i0 + it.start + min(ti, it.count - 1)];
+        const double s = w - bx.D;
+        s_s[ti] = s;
+        s_m0[ti] = (int)ceil((s - bx.a2) / bx.delta);
+        const double nu = (double)T.order + 0.5 * (double)T.half;
+        double sp, cp;
+        sincos_cw(w * bx.C, &sp, &cp);
+        const int q = (int)lround(2.0 * nu + 1.0) & 7;
+        const double c0 = cos_qpi4(q), s0 = -cos_qpi4((q + 6) & 7);
+        const double amp = bx.h * bx.delta / (bx.a1 * ft_eval(ft + T.ft_off, s * bx.a1, T.kmax));
+        s_fr[ti] = (cp * c0 - sp * s0) * amp;
+        s_fi[ti] = (sp * c0 + cp * s0) * amp;
+        s_b0[ti] = ti < it.count ? T.sign * sqrt(2.0 / kPi) / sqrt(w) : 0.0;
+        const double rw = 1.0 / (bx.rho * w);
+        double pw = 1.0;
+        for (int e = 0; e < EMAX; ++e) {
+            s_P[ti][e] = e < E ? ((((e >> 1) & 1) ? -T.a[e] : T.a[e]) * pw) : 0.0;
+            pw *= rw;
+        }
+    }
+    __syncthreads();
+    int tb = 1;
+    while (tb < T.B && tb < 16) tb <<= 1;
+    const int nsub = 16 / tb, G = (E + nsub - 1) / nsub;
+    const int ia = lane & 15, ka = lane >> 4, jb = lane & 15, tbk = jb % tb, e2b = jb / tb;
+    const int ntile = (it.count + 15) >> 4;
+    // this wave's tiles u: wave + 4 u; their phi operands and window offsets
+    R phiA[TPW][4];
+    int woff[TPW], nst[TPW];
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+        const int t = wave + 4 * u;
+        const int b0 = 16 * t, cnt = min(16, it.count - b0);
+        woff[u] = 0;
+        nst[u] = 0;
+#pragma unroll
+        for (int st = 0; st < 4; ++st) phiA[u][st] = (R)0;
+        if (t < ntile) {
+            const int mw = s_m0[b0] - 1;                 // the tile's window start
+            woff[u] = mw - it.mw0;
+            // K steps of 4 positions that hold a kernel tap of some target
+            // (the first target's taps start at mw + 1, the last's end at
+            // its m0 + kw - 1): the others are all zero and are skipped
+            const int nsteps = min(4, (s_m0[b0 + cnt - 1] + kw - mw + 3) >> 2);
+            nst[u] = nsteps;
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                const int m = mw + 4 * st + ka;
+                const double x = (s_s[b0 + ia] - (double)m * bx.delta) / bx.a2;
+                phiA[u][st] = (st < nsteps && ia < cnt) ? (R)es_kernel(x, T.beta) : (R)0;
+            }
+        }
+    }
+    // the window of (column chunk cb, exponent group g) is loaded into
+    // registers one group ahead (consecutive threads take consecutive
+    // positions of a channel row), so that its latency overlaps the previous
+    // group's products, and written to LDS between two barriers
+    constexpr int PF = 16 * kIrPos / 256;
+    C2 pf[PF];
+    int pc[PF], pp_[PF];     // channel and position of this thread's r-th value
+    {
+        const int dc = 256 / it.npos, dp = 256 - dc * it.npos;
+        int c = tid / it.npos, p = tid - c * it.npos;
+#pragma unroll
+        for (int r = 0; r < PF; ++r) {
+            pc[r] = c;
+            pp_[r] = p;
+            c += dc;
+            p += dp;
+            if (p >= it.npos) { p -= it.npos; ++c; }
+        }
+    }
+    auto load_group = [&](int cb_, int g_) {
+#pragma unroll
+        for (int r = 0; r < PF; ++r) {
+            pf[r].x = (GR)0;
+            pf[r].y = (GR)0;
+            if (pc[r] < 16) {
+                const int c = pc[r], p = pp_[r];
+                const int e = g_ * nsub + c / tb, t = cb_ + c % tb;
+                if (e < E && t < T.B) {
+                    int m = it.mw0 + p;
+                    m = m < 0 ? m + bx.Nf : (m >= bx.Nf ? m - bx.Nf : m);
+                    pf[r] = grid[bx.goff + (long long)(e * T.B + t) * bx.Nf + m];
+                }
+            }
+        }
+    };
+    load_group(0, 0);
+    for (int cb = 0; cb < T.B; cb += tb) {
+        double Sr[TPW][4], Si[TPW][4];
+#pragma unroll
+        for (int u = 0; u < TPW; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { Sr[u][v] = 0.0; Si[u][v] = 0.0; }
+        for (int g = 0; g < G; ++g) {
+            __syncthreads();   // the previous group's window is consumed
+#pragma unroll
+            for (int r = 0; r < PF; ++r)
+                if (pc[r] < 16) s_g[pc[r] * kIrStride + pp_[r]] = pf[r];
+            __syncthreads();
+            if (g + 1 < G) load_group(cb, g + 1);
+            else if (cb + tb < T.B) load_group(cb + tb, 0);
+            const int eb = g * nsub + e2b;
+            const bool eok = (cb + tbk < T.B) && eb < E;
+            const bool odd = eb & 1;
+            const int ec = eok ? eb : 0;
+#pragma unroll
+            for (int u = 0; u < TPW; ++u) {
+                const int t = wave + 4 * u;
+                if (t >= ntile) continue;
+                const C2 *row = s_g + jb * kIrStride + woff[u] + ka;
+                V dr0, di0, dr1, di1;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) { dr0[v] = (R)0; di0[v] = (R)0; dr1[v] = (R)0; di1[v] = (R)0; }
+                const C2 b0 = row[0], b1 = row[4], b2 = row[8];
+                dr0 = Mfma<R>::mma(phiA[u][0], (R)b0.x, dr0);
+                di0 = Mfma<R>::mma(phiA[u][0], (R)b0.y, di0);
+                dr1 = Mfma<R>::mma(phiA[u][1], (R)b1.x, dr1);
+                di1 = Mfma<R>::mma(phiA[u][1], (R)b1.y, di1);
+                if (nst[u] > 2) {
+                    dr0 = Mfma<R>::mma(phiA[u][2], (R)b2.x, dr0);
+                    di0 = Mfma<R>::mma(phiA[u][2], (R)b2.y, di0);
+                }
+                if (nst[u] > 3) {
+                    const C2 b3 = row[12];
+                    dr1 = Mfma<R>::mma(phiA[u][3], (R)b3.x, dr1);
+                    di1 = Mfma<R>::mma(phiA[u][3], (R)b3.y, di1);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = 16 * t + Mfma<R>::row(lane, v);
+                    const double pp = eok ? s_P[i][ec] : 0.0;
+                    const double gr = (double)dr0[v] + (double)dr1[v];
+                    const double gi = (double)di0[v] + (double)di1[v];
+                    // S += p i^(e mod 2) G
+                    Sr[u][v] = fma(pp, odd ? -gi : gr, Sr[u][v]);
+                    Si[u][v] = fma(pp, odd ? gr : gi, Si[u][v]);
+                }
+            }
+        }
+        // epilogue per tile: outputs, the exponent sub-split folded (lanes j
+        // and j + tb e2 share a column), transposed through LDS so that each
+        // atomic instruction adds 16 consecutive rows of 4 columns
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+            const int t = wave + 4 * u;
+            double out[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 16 * t + Mfma<R>::row(lane, v);
+                out[v] = t < ntile ? s_b0[i] * (s_fr[i] * Sr[u][v] - s_fi[i] * Si[u][v]) : 0.0;
+            }
+            for (int off = tb; off < 16; off <<= 1)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) out[v] += __shfl_xor(out[v], off, 64);
+            __syncthreads();
+            if (e2b == 0) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) s_out[wave][Mfma<R>::row(lane, v)][jb] = out[v];
+            }
+            __syncthreads();
+            if (t < ntile) {
+                for (int idx = lane; idx < 16 * tb; idx += 64) {
+                    const int c = idx >> 4, i = idx & 15;
+                    if (16 * t + i < it.count && cb + c < T.B)
+                        atomicAdd(&T.gs[(long long)(cb + c) * T.ld_gs + bx.i0 + it.start + 16 * t + i],
+                                  s_out[wave][i][c]);
+                }
+            }
+        }
+    }
+}
+
+// Interpolation, target-dense tiles, one tile per item (the form the runs
+// above replace by default; knob interp_mfma = 2): 16 consecutive targets whose kernel
 // supports all fall in a window of nwin <= 16 grid positions starting at
 // mw0 (item.pad holds mw0, item.tb the column chunk; the window length is
 // derived on the device).  For exponent group g (nsub = 16/tb exponents):
@@ -591,7 +812,7 @@ __global__ __launch_bounds__(SpreadSched<SP>::threads, (SP && EMAX <= 10) ? 4 : 
 // for even e and minus the imaginary part for odd e, as the reference's
 // cos / sin terms (src/expansions.c:523-531, :549-557).
 template <typename GR, typename CR, int EMAX>
-__global__ __launch_bounds__(64) void k_asy_interp_mfma(const TfDesc *__restrict__ tfs,
+__global__ __launch_bounds__(64) void k_asy_interp_tile(const TfDesc *__restrict__ tfs,
                                                         const AsyBoxD *__restrict__ boxes,
                                                         const AsyItem *__restrict__ items,
                                                         const typename Cplx<GR>::T *__restrict__ grid,

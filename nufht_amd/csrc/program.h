@@ -111,11 +111,14 @@ struct Program {
     size_t loc_lds_e = 0;
     std::vector<AsyBoxD> abox;
     AsyBoxD *d_abox = nullptr;
-    std::vector<AsyItem> aspread, ainterp, azero, adense, aimf;
+    std::vector<AsyItem> aspread, ainterp, azero, adense;
+    std::vector<InterpRun> aimf;      // matrix-core interpolation runs
+    std::vector<AsyItem> aimt;        // matrix-core interpolation tiles (knob interp_mfma 2)
     std::vector<int2> arange;
     std::vector<SpreadRange> adrange;
     AsyItem *d_aspread = nullptr, *d_ainterp = nullptr, *d_azero = nullptr, *d_adense = nullptr;
-    AsyItem *d_aimf = nullptr;
+    InterpRun *d_aimf = nullptr;
+    AsyItem *d_aimt = nullptr;
     int2 *d_arange = nullptr;
     SpreadRange *d_adrange = nullptr;
     int interp_tmax = 16;

@@ -31,6 +31,7 @@ ALTERNATIVES = [
     {"dense_cell": 1e300},          # VALU spread only
     {"dense_cell": 0.25},           # matrix-core spread for almost every box
     {"interp_mfma": 0},             # VALU interpolation only
+    {"interp_mfma": 2},             # matrix-core interpolation, one tile per item
     {"own_fft": 0},                 # rocFFT for every length
     {"dense_split": 2048, "spread_split": 1024},
     {"dense_split": 8192, "spread_split": 4096},
@@ -58,6 +59,8 @@ def test_every_knob_path_meets_tolerance(device):
             assert st["spread_sc_mfma"] == 0
         if alt.get("interp_mfma") == 0:
             assert st["interp_tc_mfma"] == 0
+        if alt.get("interp_mfma") == 2:
+            assert st["interp_tc_mfma"] > 0
         if not alt:
             assert st["spread_sc_mfma"] > 0 and st["interp_tc_mfma"] > 0
         g = p.forward(c).cpu().numpy()
