@@ -1126,15 +1126,13 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
             return -540;
         }
         if (P->loc_lds_e > 64 * 1024) {
-            HIP_TRY(hipFuncSetAttribute((const void *)k_loc_eval1,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)P->loc_lds_e));
-            HIP_TRY(hipFuncSetAttribute((const void *)k_loc_eval<16>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)P->loc_lds_e));
-            HIP_TRY(hipFuncSetAttribute((const void *)k_loc_eval<32>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)P->loc_lds_e));
+            for (const void *f : {(const void *)k_loc_eval1<double>, (const void *)k_loc_eval1<float>})
+                HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)P->loc_lds_e));
+            for (const void *f : {(const void *)k_loc_eval<16, double>, (const void *)k_loc_eval<32, double>,
+                                  (const void *)k_loc_eval<16, float>, (const void *)k_loc_eval<32, float>})
+                HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)P->loc_lds_e));
         }
         for (size_t b = 0; b < P->lbox.size(); ++b) {
             const TfDesc &T = P->tfs[P->lbox[b].tf];
@@ -1278,10 +1276,16 @@ static void launch_asy_e(Program *P, hipStream_t st)
     }
 }
 
+// Local-expansion contractions in FP32 in mixed precision (the Bessel
+// products b_l and the basis recurrence stay FP64), FP64 otherwise.
 template <int NLT> static void launch_loc_proj_n(Program *P, hipStream_t st, double *coef)
 {
-    hipLaunchKernelGGL(k_loc_proj<NLT>, dim3((unsigned)P->lproj.size()), dim3(256),
-                       LocProjLds<NLT>::bytes, st, P->d_tfs, P->d_lbox, P->d_lproj, coef);
+    if (P->prec == 2)
+        hipLaunchKernelGGL((k_loc_proj<NLT, float>), dim3((unsigned)P->lproj.size()), dim3(256),
+                           LocProjLds<NLT>::bytes, st, P->d_tfs, P->d_lbox, P->d_lproj, coef);
+    else
+        hipLaunchKernelGGL((k_loc_proj<NLT, double>), dim3((unsigned)P->lproj.size()), dim3(256),
+                           LocProjLds<NLT>::bytes, st, P->d_tfs, P->d_lbox, P->d_lproj, coef);
 }
 
 static void launch_loc_proj(Program *P, hipStream_t st, double *coef)
@@ -1320,19 +1324,26 @@ static int run_program(Program *P, hipStream_t st, double *coef)
         }
         {
             StageTimer tm(ST_LOCE, st);
-            if (!P->leval1.empty())
-                hipLaunchKernelGGL(k_loc_eval1, dim3((unsigned)P->leval1.size()), dim3(P->loc_tpb),
-                                   P->loc_lds_e, st, P->d_tfs, P->d_lbox, P->d_leval1, coef,
-                                   P->d_cheb);
-            if (P->leval.empty()) {
-            } else if (P->loc_bmax == 32)
-                hipLaunchKernelGGL(k_loc_eval<32>, dim3((unsigned)P->leval.size()),
-                                   dim3(P->loc_tpb), P->loc_lds_e, st, P->d_tfs, P->d_lbox,
-                                   P->d_leval, P->d_lev_boxes, coef, P->d_cheb);
-            else
-                hipLaunchKernelGGL(k_loc_eval<16>, dim3((unsigned)P->leval.size()),
-                                   dim3(P->loc_tpb), P->loc_lds_e, st, P->d_tfs, P->d_lbox,
-                                   P->d_leval, P->d_lev_boxes, coef, P->d_cheb);
+            if (!P->leval1.empty()) {
+                if (P->prec == 2)
+                    hipLaunchKernelGGL(k_loc_eval1<float>, dim3((unsigned)P->leval1.size()),
+                                       dim3(P->loc_tpb), P->loc_lds_e, st, P->d_tfs, P->d_lbox,
+                                       P->d_leval1, coef, P->d_cheb);
+                else
+                    hipLaunchKernelGGL(k_loc_eval1<double>, dim3((unsigned)P->leval1.size()),
+                                       dim3(P->loc_tpb), P->loc_lds_e, st, P->d_tfs, P->d_lbox,
+                                       P->d_leval1, coef, P->d_cheb);
+            }
+            if (!P->leval.empty()) {
+                const bool f32 = P->prec == 2;
+                const void *fn = P->loc_bmax == 32
+                                     ? (f32 ? (const void *)k_loc_eval<32, float> : (const void *)k_loc_eval<32, double>)
+                                     : (f32 ? (const void *)k_loc_eval<16, float> : (const void *)k_loc_eval<16, double>);
+                void *args[] = {(void *)&P->d_tfs, (void *)&P->d_lbox, (void *)&P->d_leval,
+                                (void *)&P->d_lev_boxes, (void *)&coef, (void *)&P->d_cheb};
+                HIP_TRY(hipLaunchKernel(fn, dim3((unsigned)P->leval.size()), dim3(P->loc_tpb), args,
+                                        P->loc_lds_e, st));
+            }
         }
     }
     if (!P->dir.empty()) {

@@ -156,36 +156,41 @@ template <int NLT> struct LocProjLds {
     static constexpr size_t bytes = (stage > red ? stage : red) * sizeof(double);
 };
 
-template <int NLT>
+// LR: the contraction's precision (double, or float in mixed precision: the
+// basis is computed in FP64 and rounded, the products and their sums over
+// the item's sources are FP32, the per-item partial sums are added to the
+// FP64 moments).
+template <int NLT, typename LR>
 __global__ __launch_bounds__(256) void k_loc_proj(const TfDesc *__restrict__ tfs,
                                                   const LocBoxD *__restrict__ boxes,
                                                   const LocItem *__restrict__ items,
                                                   double *__restrict__ coef)
 {
     constexpr int LT = LocProjLds<NLT>::LT, LC = kLocBM + 1;
-    extern __shared__ double sm[];
-    double *sT = sm;                          // [kLocSub][LT]
-    double *sC = sm + (size_t)kLocSub * LT;   // [kLocSub][LC]
+    extern __shared__ double sm_raw[];
+    LR *sm = (LR *)sm_raw;
+    LR *sT = sm;                              // [kLocSub][LT]
+    LR *sC = sm + (size_t)kLocSub * LT;       // [kLocSub][LC]
     LocItem it = items[blockIdx.x];
     LocBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
     const int K = T.K_loc, par = T.order & 1;
     const int nb = min(kLocBM, T.B - it.col0);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    mf_d4 d[NLT];
+    typename Mfma<LR>::V d[NLT];
 #pragma unroll
     for (int lt = 0; lt < NLT; ++lt)
-        for (int v = 0; v < 4; ++v) d[lt][v] = 0.0;
+        for (int v = 0; v < 4; ++v) d[lt][v] = (LR)0;
     for (int s0 = 0; s0 < it.count; s0 += kLocSub) {
         const int n = min(kLocSub, it.count - s0);
         __syncthreads();
         if (tid < kLocSub) {
-            double *row = sT + (size_t)tid * LT;
+            LR *row = sT + (size_t)tid * LT;
             if (tid < n) {
                 double x = T.r[bx.j0 + it.start + s0 + tid] / bx.r_end;
                 x = fmin(1.0, fmax(-1.0, x));
                 double tm1 = 1.0, t0 = x;   // T_0, T_1
-                row[0] = par ? 2.0 * x : 1.0;
+                row[0] = (LR)(par ? 2.0 * x : 1.0);
                 int deg = 1;
                 for (int l = 1; l < 16 * NLT; ++l) {
                     if (l <= K) {
@@ -194,13 +199,13 @@ __global__ __launch_bounds__(256) void k_loc_proj(const TfDesc *__restrict__ tfs
                             tm1 = t0;
                             t0 = tn;
                         }
-                        row[l] = 2.0 * t0;
+                        row[l] = (LR)(2.0 * t0);
                     } else {
-                        row[l] = 0.0;
+                        row[l] = (LR)0;
                     }
                 }
             } else {
-                for (int l = 0; l < 16 * NLT; ++l) row[l] = 0.0;
+                for (int l = 0; l < 16 * NLT; ++l) row[l] = (LR)0;
             }
         }
         // coefficients: lanes run over sources, 256 / kLocSub columns at a time
@@ -212,37 +217,36 @@ __global__ __launch_bounds__(256) void k_loc_proj(const TfDesc *__restrict__ tfs
                 c = T.cs[(long long)(it.col0 + t) * T.ld_cs + j];
                 if (T.half) c *= sqrt(T.r[j]);
             }
-            sC[q * LC + t] = c;
+            sC[q * LC + t] = (LR)c;
         }
         __syncthreads();
         const int ia = lane & 15, ka = lane >> 4;
         for (int st = wave; 4 * st < n; st += 4) {
             const int q = 4 * st + ka;
-            const double bv = sC[q * LC + ia];
+            const LR bv = sC[q * LC + ia];
 #pragma unroll
             for (int lt = 0; lt < NLT; ++lt)
-                d[lt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sT[q * LT + 16 * lt + ia], bv, d[lt],
-                                                            0, 0, 0);
+                d[lt] = Mfma<LR>::mma(sT[q * LT + 16 * lt + ia], bv, d[lt]);
         }
     }
     // sum the 4 waves' tiles: red[wave][16 NLT][LC]
     __syncthreads();
-    double *red = sm;
+    LR *red = sm;
 #pragma unroll
     for (int lt = 0; lt < NLT; ++lt)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            int row = 16 * lt + (lane >> 4) + 4 * v;
+            int row = 16 * lt + Mfma<LR>::row(lane, v);
             red[((size_t)wave * 16 * NLT + row) * LC + (lane & 15)] = d[lt][v];
         }
     __syncthreads();
     for (int e = tid; e < 16 * NLT * 16; e += 256) {
         int row = e >> 4, t = e & 15;
         if (row > K || t >= nb) continue;
-        double acc = 0.0;
+        LR acc = (LR)0;
 #pragma unroll
         for (int w = 0; w < 4; ++w) acc += red[((size_t)w * 16 * NLT + row) * LC + t];
-        atomicAdd(&coef[bx.coef_off + (long long)row * T.B + it.col0 + t], acc);
+        atomicAdd(&coef[bx.coef_off + (long long)row * T.B + it.col0 + t], (double)acc);
     }
 }
 
@@ -267,6 +271,7 @@ __host__ __device__ inline int loc_slots(int order, int half, int K)
     return half ? 2 * (K + 1) : (order >> 1) + K + (order & 1) + 1;
 }
 
+template <typename LR>
 __global__ __launch_bounds__(256) void k_loc_eval1(const TfDesc *__restrict__ tfs, const LocBoxD *__restrict__ boxes,
                            const LocItem *__restrict__ items, const double *__restrict__ coef,
                            const double *__restrict__ cheb)
@@ -342,24 +347,24 @@ __global__ __launch_bounds__(256) void k_loc_eval1(const TfDesc *__restrict__ tf
     const int B = T.B;
     for (int t0 = 0; t0 < B; t0 += 4) {
         const int nt = min(4, B - t0);
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        LR a0 = (LR)0, a1 = (LR)0, a2 = (LR)0, a3 = (LR)0;
         for (int l = 0; l <= K; ++l) {
-            double bl = S[(boff + l) * nthr];
+            const LR bl = (LR)S[(boff + l) * nthr];
             const double *cr = cf + (long long)l * B + t0;
-            a0 = fma(bl, cr[0], a0);
-            if (nt > 1) a1 = fma(bl, cr[1], a1);
-            if (nt > 2) a2 = fma(bl, cr[2], a2);
-            if (nt > 3) a3 = fma(bl, cr[3], a3);
+            a0 = fma(bl, (LR)cr[0], a0);
+            if (nt > 1) a1 = fma(bl, (LR)cr[1], a1);
+            if (nt > 2) a2 = fma(bl, (LR)cr[2], a2);
+            if (nt > 3) a3 = fma(bl, (LR)cr[3], a3);
         }
         double *g = T.gs + (long long)t0 * T.ld_gs + i;
-        atomicAdd(g, sc * a0);
-        if (nt > 1) atomicAdd(g + T.ld_gs, sc * a1);
-        if (nt > 2) atomicAdd(g + 2 * (long long)T.ld_gs, sc * a2);
-        if (nt > 3) atomicAdd(g + 3 * (long long)T.ld_gs, sc * a3);
+        atomicAdd(g, sc * (double)a0);
+        if (nt > 1) atomicAdd(g + T.ld_gs, sc * (double)a1);
+        if (nt > 2) atomicAdd(g + 2 * (long long)T.ld_gs, sc * (double)a2);
+        if (nt > 3) atomicAdd(g + 3 * (long long)T.ld_gs, sc * (double)a3);
     }
 }
 
-template <int BMAX>
+template <int BMAX, typename LR>
 __global__ __launch_bounds__(256) void k_loc_eval(const TfDesc *__restrict__ tfs, const LocBoxD *__restrict__ boxes,
                            const LocEvalItem *__restrict__ items,
                            const int *__restrict__ boxlist, const double *__restrict__ coef,
@@ -375,9 +380,9 @@ __global__ __launch_bounds__(256) void k_loc_eval(const TfDesc *__restrict__ tfs
     const int i = it.i0 + tid;
     const double w = T.w[i];
     const int B = T.B, nb = min(BMAX, B - it.col0);
-    double acc[BMAX];
+    LR acc[BMAX];
 #pragma unroll
-    for (int t = 0; t < BMAX; ++t) acc[t] = 0.0;
+    for (int t = 0; t < BMAX; ++t) acc[t] = (LR)0;
     for (int q = 0; q < it.rcnt; ++q) {
         const LocBoxD bx = boxes[boxlist[it.roff + q]];
         int boff;               // slot of b_0 (b_l at slot boff + l)
@@ -436,18 +441,18 @@ __global__ __launch_bounds__(256) void k_loc_eval(const TfDesc *__restrict__ tfs
         }
         const double *cf = coef + bx.coef_off + it.col0;
         for (int l = 0; l <= K; ++l) {
-            const double bl = S[(boff + l) * nthr];
+            const LR bl = (LR)S[(boff + l) * nthr];
             const double *cr = cf + (long long)l * B;
 #pragma unroll
             for (int t = 0; t < BMAX; ++t)
-                if (t < nb) acc[t] = fma(bl, cr[t], acc[t]);
+                if (t < nb) acc[t] = fma(bl, (LR)cr[t], acc[t]);
         }
     }
     const double sc = T.sign * (T.half ? sqrt(2.0 * w / kPi) : 1.0);
     double *g = T.gs + (long long)it.col0 * T.ld_gs + i;
 #pragma unroll
     for (int t = 0; t < BMAX; ++t)
-        if (t < nb) atomicAdd(g + (long long)t * T.ld_gs, sc * acc[t]);
+        if (t < nb) atomicAdd(g + (long long)t * T.ld_gs, sc * (double)acc[t]);
 }
 
 
