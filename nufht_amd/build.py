@@ -57,7 +57,7 @@ def _jobs() -> int:
 # stage is compiled once per grid precision and exponent bound so that its
 # kernel instantiations build in parallel
 def _hip_units():
-    units = [("engine.hip", "engine.o", [])]
+    units = [("engine.hip", "engine.o", []), ("scratch.hip", "scratch.o", [])]
     for rname, tag in (("double", "f64"), ("float", "f32")):
         units.append(("fft_stage.hip", f"fft_{tag}.o", [f"-DNUFHT_ASY_R={rname}"]))
         for em in (6, 10, 14, 22):

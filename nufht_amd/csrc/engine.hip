@@ -957,9 +957,18 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         FftClass c;
         c.nf = kv.first;
         c.off = off;
-        c.own = P->knobs.own_fft &&
-                fft_factor(c.nf, mixed ? fft_max_elems<float>() : fft_max_elems<double>(), c.p1,
-                           c.p2);
+        const int maxe = mixed ? fft_max_elems<float>() : fft_max_elems<double>();
+        c.own = P->knobs.own_fft && fft_factor(c.nf, maxe, c.p1, c.p2);
+        if (P->knobs.own_fft && !c.own && c.nf <= 4096 && c.nf <= maxe) {
+            // one pass: M whole channels of Nf points per block
+            c.own = c.small = true;
+            c.p1.N = c.p1.nf = c.nf;
+            c.p1.M = 1;
+            while (c.p1.M < 16 && 2 * c.p1.M * c.nf <= maxe) c.p1.M <<= 1;
+            c.p1.lgM = 0;
+            while ((1 << c.p1.lgM) < c.p1.M) ++c.p1.lgM;
+            fft_radices(c.nf, c.p1);
+        }
         own_nf[c.nf] = c.own;
         c.p1.chinfo_off = c.p2.chinfo_off = (int)P->fft_chinfo.size();
         for (int q : kv.second) {
@@ -987,6 +996,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 P->stats.fft_bytes += (double)ch * (mixed ? 8.0 : 16.0) * 2.0 * A.Nf;
             }
         }
+        c.p1.nch = c.p2.nch = (int)c.batch;
         P->fft.push_back(c);
     }
     P->grid_elems = off;
@@ -1191,6 +1201,17 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     for (auto &c : P->fft) {
         if (!c.own) continue;
         fft_table(tw, c.p1.N, c.p1.N, &c.p1.w_off);
+        if (c.small) {
+            FftPass &fp = c.p1;
+            fp.lds_stride = fp.M == 1 ? 1 : fp.M + 1;
+            fp.lds_shift = 3;
+            if (fft_lds_elems(fp.N, fp.lds_stride, fp.lds_shift, fp.M) * esz > 80 * 1024) {
+                fp.lds_stride = fp.M;
+                fp.lds_shift = 2;
+            }
+            c.lds1 = fft_lds_elems(fp.N, fp.lds_stride, fp.lds_shift, fp.M) * esz;
+            continue;
+        }
         fft_table(tw, c.p2.N, c.p2.N, &c.p2.w_off);
         // w^a = hi[a >> 8] lo[a & 255], w = e^{2 pi i / Nf}
         c.p1.hi_off = (int)(tw.size() / 2);
@@ -1468,37 +1489,8 @@ extern "C" int nufht_batch_device(const NufhtPlan *plan, const double *d_cs, int
 // Allocation-free execution (nufht_batch_device_noalloc).  The kernels'
 // scratch, i.e. the local-expansion moments (partial sums, cleared by a fill
 // kernel at the start of every call) and the sorted-order staging of
-// callers whose grids are not ascending, lives in this file's static device
-// array instead of buffers the library allocates.  Each GPU the library
-// runs on holds its own copy of the array until the process ends; host code
-// reaches it only through the per-device address from hipGetSymbolAddress.
-constexpr long long kScratchDoubles = 1LL << 24;   // 128 MiB per GPU
-static __device__ __align__(256) double scratch[kScratchDoubles];
-constexpr int kScratchMaxDevices = 64;
-static std::mutex g_scratch_mu;
-static double *g_scratch_ptr[kScratchMaxDevices];
-
-// Device address of the current device's scratch array (looked up once per
-// device, with that device current).
-static int scratch_ptr(double **out)
-{
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    if (dev < 0 || dev >= kScratchMaxDevices) {
-        nufht_set_error("device %d is beyond the %d the scratch table covers", dev,
-                        kScratchMaxDevices);
-        return -550;
-    }
-    std::lock_guard<std::mutex> g(g_scratch_mu);
-    if (!g_scratch_ptr[dev]) {
-        void *p = nullptr;
-        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
-        g_scratch_ptr[dev] = (double *)p;
-    }
-    *out = g_scratch_ptr[dev];
-    return 0;
-}
-
+// callers whose grids are not ascending, lives in the static device array of
+// scratch.hip instead of buffers the library allocates.
 extern "C" int nufht_batch_device_noalloc(const NufhtPlan *plan, const double *d_cs, int ld_cs,
                                           double *d_gs, int ld_gs, int batch, void *stream)
 {
@@ -1507,10 +1499,13 @@ extern "C" int nufht_batch_device_noalloc(const NufhtPlan *plan, const double *d
     NufhtImplHost *h = (NufhtImplHost *)plan->impl;
     const int m = plan->m, n = plan->n, B = batch;
     const long long n_in = h->perm_r ? (long long)m * B : 0, n_out = h->perm_w ? (long long)n * B : 0;
-    if (n_in + n_out > kScratchDoubles) {
+    double *base;
+    long long cap;
+    RC_TRY(nufht_scratch_array(&base, &cap));
+    if (n_in + n_out > cap) {
         nufht_set_error("nufht_batch_device_noalloc: staging of %lld doubles exceeds the scratch "
                         "array's %lld (m, n = %d, %d unsorted nodes, batch %d)",
-                        n_in + n_out, kScratchDoubles, m, n, B);
+                        n_in + n_out, cap, m, n, B);
         return -550;
     }
     EngineState *E = (EngineState *)h->dev;
@@ -1528,14 +1523,12 @@ extern "C" int nufht_batch_device_noalloc(const NufhtPlan *plan, const double *d
         return -551;
     }
     Program *P = E->prog;
-    if (P->ncoef + n_in + n_out > kScratchDoubles) {
+    if (P->ncoef + n_in + n_out > cap) {
         nufht_set_error("nufht_batch_device_noalloc: %lld local-expansion moments and %lld "
                         "staged values exceed the scratch array's %lld doubles",
-                        P->ncoef, n_in + n_out, kScratchDoubles);
+                        P->ncoef, n_in + n_out, cap);
         return -550;
     }
-    double *base;
-    RC_TRY(scratch_ptr(&base));
     double *coef = base, *s_in = base + P->ncoef, *s_out = s_in + n_in;
     hipStream_t st = (hipStream_t)stream;
     RC_TRY(order_after_last(E, st));

@@ -306,4 +306,42 @@ __global__ __launch_bounds__(kFftThreads, 4) void k_fft_rows(const typename Cplx
     }
 }
 
+// Lengths up to one tile (Nf <= 4096 in FP64, 8192 in FP32): one pass, M
+// whole channels per block (each a contiguous row), the same Stockham stages
+// in LDS.  Grid positions outside [-L, L] (mod Nf) are zeros the spread did
+// not store; only the frequencies mlo .. mlo + span (mod Nf) are written.
+template <typename R>
+__global__ __launch_bounds__(kFftThreads) void k_fft_small(const typename Cplx<R>::T *__restrict__ g,
+                                                           typename Cplx<R>::T *__restrict__ out,
+                                                           FftPass P,
+                                                           const typename Cplx<R>::T *__restrict__ tw,
+                                                           const int4 *__restrict__ chinfo)
+{
+    typedef typename Cplx<R>::T C2;
+    extern __shared__ unsigned char s_raw[];
+    C2 *s = (C2 *)s_raw;
+    const int N = P.N, M = P.M, ch0 = blockIdx.x * M;
+    for (int idx = threadIdx.x; idx < M * N; idx += kFftThreads) {
+        const int c = idx / N, n = idx - c * N, ch = ch0 + c;
+        C2 v;
+        v.x = (R)0;
+        v.y = (R)0;
+        if (ch < P.nch) {
+            const int L = chinfo[P.chinfo_off + ch].x;
+            if (n <= L || n >= N - L) v = g[(long long)ch * N + n];
+        }
+        s[fft_lds_idx(n, c, P.lds_stride, P.lds_shift)] = v;
+    }
+    __syncthreads();
+    fft_lds<R>(s, P, tw + P.w_off);
+    for (int idx = threadIdx.x; idx < M * N; idx += kFftThreads) {
+        const int c = idx / N, n = idx - c * N, ch = ch0 + c;
+        if (ch >= P.nch) continue;
+        const int4 ci = chinfo[P.chinfo_off + ch];
+        int d = n - ci.y;
+        d = d < 0 ? d + N : (d >= N ? d - N : d);
+        if (d <= ci.z) out[(long long)ch * N + n] = s[fft_lds_idx(n, c, P.lds_stride, P.lds_shift)];
+    }
+}
+
 }  // namespace nufht_dev

@@ -29,6 +29,8 @@ struct FftPass {
                               // l in [-L, L] (the rest are zeros not stored), pass 2
                               // writes only frequencies mlo .. mlo + span (mod Nf)
     long long nf;
+    int nch;                  // single-pass classes: channels of the class
+    int pad_;
 };
 
 __host__ __device__ inline size_t fft_lds_elems(int N, int stride, int shift, int M)

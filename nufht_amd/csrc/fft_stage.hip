@@ -34,10 +34,18 @@ void asy_fft(Program *P, hipStream_t st)
                             160 * 1024);
         hipFuncSetAttribute((const void *)k_fft_rows<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             160 * 1024);
+        hipFuncSetAttribute((const void *)k_fft_small<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            160 * 1024);
     });
     StageTimer tm(ST_FFT, st);
     for (auto &c : P->fft) {
         G *in = (G *)P->d_grid + c.off, *out = (G *)P->d_grid2 + c.off;
+        if (c.own && c.small) {
+            hipLaunchKernelGGL((k_fft_small<R>), dim3((unsigned)((c.batch + c.p1.M - 1) / c.p1.M)),
+                               dim3(kFftThreads), c.lds1, st, (const G *)in, out, c.p1,
+                               (const G *)P->d_fftw, (const int4 *)P->d_fft_chinfo);
+            continue;
+        }
         if (c.own) {
             const G *tw = (const G *)P->d_fftw;
             hipLaunchKernelGGL((k_fft_cols<R>), dim3((unsigned)(c.batch * c.p1.tiles)),

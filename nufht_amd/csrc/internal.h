@@ -51,6 +51,10 @@ int nufht_build_partition(const double *r, int m, const double *w, int n,
                           double z, int max_levels, int min_dim_prod, int kloc,
                           Boxes *out);
 
+/* scratch.hip: the allocation-free entry point's static device array (its
+ * address on the current device and its length in doubles) */
+int  nufht_scratch_array(double **out, long long *count);
+
 /* engine.hip */
 void nufht_engine_release(NufhtImplHost *h);
 int  nufht_engine_batch_host(const NufhtPlan *plan, const double *cs, int ld_cs,
