@@ -17,12 +17,19 @@
  * clamps to 10 get z_split = 0 and K_loc = 0.  Slot 10 is filled here by the
  * same Newton solve with K = 11.
  *
- * If NUFHT_TABLE_CACHE names a file in the reference's cache format
- * (src/fast_hankel_nufht.c:186-247) its z and K entries are used instead,
- * which reproduces the reference's boxes bit for bit.
+ * A cache file in the reference's format (src/fast_hankel_nufht.c:186-247),
+ * found where the reference looks for it ($NUFHT_TABLE_CACHE, then
+ * data/nufht_tables.cache under the working directory or its parent, then
+ * next to the executable), supplies its z and K entries instead, which
+ * reproduces the reference's boxes bit for bit.  Unlike the reference,
+ * planning never writes the file: nufht_amd_write_table_cache does, on
+ * request.
  */
+#include <limits.h>
 #include <math.h>
 #include <pthread.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -105,10 +112,60 @@ static int wimp_terms(double nu, double z, double tol)
     return k;
 }
 
+/* The cache file to read, in the reference's order of preference
+ * (src/fast_hankel_nufht.c:95-160): $NUFHT_TABLE_CACHE; else
+ * data/nufht_tables.cache under the working directory, if it has a data/
+ * directory, or under its parent, if that has one; else the same name next
+ * to the executable.  Returns 0 and the path in out, or -1. */
+static int cache_path(char *out, size_t outlen)
+{
+    const char *env = getenv("NUFHT_TABLE_CACHE");
+    if (env) {
+        if (!*env || strlen(env) + 1 > outlen) return -1;
+        strcpy(out, env);
+        return 0;
+    }
+    const char *fname = "data/nufht_tables.cache";
+    char cwd[PATH_MAX];
+    struct stat st;
+    if (getcwd(cwd, sizeof(cwd))) {
+        for (int up = 0; up < 2; ++up) {
+            char dir[PATH_MAX];
+            size_t len = strlen(cwd);
+            if (up) {
+                const char *slash = strrchr(cwd, '/');
+                if (!slash) break;
+                len = (size_t)(slash - cwd);
+            }
+            if (len + 6 >= sizeof(dir) || len + 1 + strlen(fname) + 1 > outlen) continue;
+            memcpy(dir, cwd, len);
+            strcpy(dir + len, "/data");
+            if (stat(dir, &st) == 0 && S_ISDIR(st.st_mode)) {
+                memcpy(out, cwd, len);
+                out[len] = '/';
+                strcpy(out + len + 1, fname);
+                return 0;
+            }
+        }
+    }
+    char exe[PATH_MAX];
+    ssize_t r = readlink("/proc/self/exe", exe, sizeof(exe) - 1);
+    if (r <= 0) return -1;
+    exe[r] = '\0';
+    const char *slash = strrchr(exe, '/');
+    if (!slash) return -1;
+    size_t len = (size_t)(slash - exe);
+    if (len == 0 || len + 1 + strlen(fname) + 1 > outlen) return -1;
+    memcpy(out, exe, len);
+    out[len] = '/';
+    strcpy(out + len + 1, fname);
+    return 0;
+}
+
 static void try_cache(void)
 {
-    const char *path = getenv("NUFHT_TABLE_CACHE");
-    if (!path || !*path) return;
+    char path[PATH_MAX];
+    if (cache_path(path, sizeof(path))) return;
     FILE *f = fopen(path, "rb");
     if (!f) return;
     uint32_t hdr[6];

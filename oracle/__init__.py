@@ -62,6 +62,7 @@ def lib():
         L.orc_nufht_batch.argtypes = [ctypes.c_double, _DP, ctypes.c_int, _DP, ctypes.c_int,
                                       _DP, ctypes.c_int, _DP, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_set_plant.argtypes = [ctypes.c_int]
         L.orc_nufft3.argtypes = [ctypes.c_int, _DP, _DP, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_int, _DP, _DP, ctypes.c_int]
         L.orc_nufft3_direct.argtypes = [ctypes.c_int, _DP, _DP, ctypes.c_int, ctypes.c_int,

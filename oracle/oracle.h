@@ -74,6 +74,8 @@ void orc_split_box(const double *rs, int m, const double *ws, int n,
  * asy_mode 0: exact exponential sums in the asymptotic regime (pins the
  * algebra, O(n_i n_j)); 1: the oracle's own CPU type-3 NUFFT at eps = tol
  * (the timed CPU baseline).  Returns the reference's error codes. */
+/* planted mistakes of the port (tests only; 0 = none, see port.c) */
+void orc_set_plant(int which);
 int orc_nufht_batch(double nu, const double *rs, int m, const double *ws,
                     int n, const double *cs, int ldc, double *gs, int ldg,
                     int B, double tol, int asy_mode, int nthreads);

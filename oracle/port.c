@@ -24,6 +24,14 @@
 
 #include "oracle.h"
 
+/* Planted mistakes, for the tests that show the direct-sum comparison has
+ * teeth (0 = none): 1 the odd (sine) Hankel terms' sign flipped, 2 the odd
+ * terms' source weight r^(-2l-3/2) taken as r^(-2l-1/2), 3 the Wimp
+ * expansion's alternating sign (-1)^(l-l0) dropped, 4 the Hankel phase
+ * e^{i pi (-nu/2 - 1/4)} taken as e^{i pi (-nu/2 + 1/4)}. */
+static int g_plant = 0;
+void orc_set_plant(int which) { g_plant = which; }
+
 static void add_dir_cols(double *gs, int ldg, double nu, const double *rs,
                          const double *cs, int ldc, const double *ws, int m,
                          int n, int B)
@@ -70,7 +78,7 @@ static void add_loc_cols(double *gs, int ldg, int nu, const double *rs,
             int lo = l0 - l;
             if (lo < 0) lo = -lo;
             double v = b2[l0 + odd + l] * b2[lo];
-            if (l > l0 && ((l - l0) & 1)) v = -v;
+            if (l > l0 && ((l - l0) & 1) && g_plant != 3) v = -v;
             b1[l] = v;
         }
         for (int t = 0; t < B; ++t) {
@@ -88,16 +96,17 @@ static void add_asy_cols(double *gs, int ldg, double nu, const double *rs,
                          int mode)
 {
     double sq = sqrt(2.0 / M_PI);
-    double pa = M_PI * (-nu / 2.0 - 0.25);
+    double pa = M_PI * (-nu / 2.0 + (g_plant == 4 ? 0.25 : -0.25));
     double complex ph = cos(pa) + I * sin(pa);
     double *cin = malloc(sizeof(double) * 2 * (size_t)m * B);
     double *cout = malloc(sizeof(double) * 2 * (size_t)n * B);
     for (int l = 0; l <= K; ++l) {
         for (int half = 0; half < 2; ++half) {
             double ex = -2.0 * l - 1.0 - half + 0.5;
+            double exs = (g_plant == 2 && half) ? ex + 1.0 : ex;
             for (int t = 0; t < B; ++t)
                 for (int k = 0; k < m; ++k) {
-                    cin[2 * ((size_t)t * m + k)] = cs[k + (size_t)t * ldc] * pow(rs[k], ex);
+                    cin[2 * ((size_t)t * m + k)] = cs[k + (size_t)t * ldc] * pow(rs[k], exs);
                     cin[2 * ((size_t)t * m + k) + 1] = 0.0;
                 }
             if (mode == 0) orc_nufft3_direct(m, rs, cin, +1, n, ws, cout, B);
@@ -109,7 +118,7 @@ static void add_asy_cols(double *gs, int ldg, double nu, const double *rs,
                                         I * cout[2 * ((size_t)t * n + i) + 1]) * ph;
                     double sc = coeff * pow(ws[i], ex);
                     if (!half) gs[i + (size_t)t * ldg] += creal(v) * sc;
-                    else gs[i + (size_t)t * ldg] -= cimag(v) * sc;
+                    else gs[i + (size_t)t * ldg] += (g_plant == 1 ? 1.0 : -1.0) * cimag(v) * sc;
                 }
         }
     }

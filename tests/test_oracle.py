@@ -195,15 +195,29 @@ def test_port_against_direct_sum(nu, tol):
     assert rel_l2(fast, ref) < 2 * tol
 
 
-def test_port_planted_mistake_is_caught():
-    """A dropped sign in the Hankel series (the odd-term part) must fail the
-    direct-sum comparison: the check has teeth."""
+@pytest.mark.parametrize("plant,what", [(1, "odd Hankel terms' sign flipped"),
+                                         (2, "odd terms' source weight exponent off by one"),
+                                         (3, "Wimp expansion's alternating sign dropped"),
+                                         (4, "Hankel phase's quarter-turn sign flipped")])
+def test_port_planted_mistake_is_caught(plant, what):
+    """A plausible one-symbol mistake planted in the oracle's restatement of
+    the reference's expansions (src/expansions.c:549-557, :523-531, :301-318)
+    must fail the direct-sum comparison the oracle is pinned by, while the
+    same case passes without it."""
+    lib = oracle.lib()
     s = W.Stream(12, 6)
-    r = np.sort(s.uniform(800, 0.0, 100.0))
-    k = np.sort(s.uniform(700, 0.0, 100.0))
-    cs = s.normal(800)[:, None]
-    ref = oracle.direct(2.0, r, cs, k)
-    good = oracle.nufht_batch(2.0, r, cs, k, tol=1e-8, asy_mode=0)
-    bad = oracle.nufht_batch(3.0, r, cs, k, tol=1e-8, asy_mode=0)  # wrong order
-    assert rel_l2(good, ref) < 1e-8
-    assert rel_l2(bad, ref) > 1e-2
+    r = np.sort(s.uniform(3000, 0.0, 100.0))
+    k = np.sort(s.uniform(2500, 0.0, 100.0))
+    cs = s.normal(3000)[:, None]
+    prm = oracle.params(4.0)[1]
+    bx = oracle.boxes(r, k, prm["z_split"], K_loc=prm["K_loc"])
+    assert len(bx["loc"]) and len(bx["asy"])      # both expansions contribute
+    ref = oracle.direct(4.0, r, cs, k)
+    good = oracle.nufht_batch(4.0, r, cs, k, tol=1e-8, asy_mode=0)
+    assert rel_l2(good, ref) < 2e-8
+    lib.orc_set_plant(plant)
+    try:
+        bad = oracle.nufht_batch(4.0, r, cs, k, tol=1e-8, asy_mode=0)
+    finally:
+        lib.orc_set_plant(0)
+    assert rel_l2(bad, ref) > 1e-6, what

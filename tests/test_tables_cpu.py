@@ -52,3 +52,34 @@ def test_plan_parameters_with_written_cache(tmp_path):
     cached = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
                             check=True).stdout
     assert plain.strip() and plain == cached
+
+
+def test_cache_found_where_the_reference_looks(tmp_path):
+    """Without NUFHT_TABLE_CACHE the cache is read from data/ under the
+    working directory, else under its parent (src/fast_hankel_nufht.c:95-160):
+    a cache whose z entry for (nu = 5, tol 1e-8, K_asy = 4) is altered shows
+    up in the plan built there, and nowhere else."""
+    path = str(tmp_path / "ref.cache")
+    tables.write(path)
+    raw = bytearray(open(path, "rb").read())
+    # asy_z[5][j = 4 (tol 1e-8)][slot 4] <- 12.5
+    off = 24 + 401 * 64 * 8 + ((5 * 32 + 4) * 16 + 4) * 8
+    raw[off:off + 8] = np.float64(12.5).tobytes()
+    code = ("import numpy as np, nufht_amd as na\n"
+            "p = na.Plan(5.0, np.linspace(0.1, 50, 300), np.linspace(0.1, 50, 200), tol=1e-8)\n"
+            "print(repr(p.z_split))\n")
+    env = dict(os.environ, PYTHONPATH=ROOT)
+    env.pop("NUFHT_TABLE_CACHE", None)
+    outs = {}
+    for where in ("cwd", "parent", "none"):
+        base = tmp_path / where
+        work = base / "work"
+        work.mkdir(parents=True)
+        if where != "none":
+            d = (work if where == "cwd" else base) / "data"
+            d.mkdir()
+            open(d / "nufht_tables.cache", "wb").write(raw)
+        outs[where] = float(subprocess.run([sys.executable, "-c", code], env=env, cwd=str(work),
+                                           capture_output=True, text=True, check=True).stdout)
+    assert outs["cwd"] == 12.5 and outs["parent"] == 12.5
+    assert abs(outs["none"] - 11.666739153830589) < 1e-9
