@@ -107,8 +107,10 @@ typedef struct {
 } NufhtScratch;
 
 /** Replaces reference src/fast_hankel_nufht.c:759-786.
- *  Returns 0, -200 (bad grids), -1 (tol < 1e-15), -2 (unsupported nu),
- *  -3..-6 (table index), -100 (NULL options path), -501 (out of memory). */
+ *  Returns 0, -200 (bad grids), -1 (tol < 1e-15), -2 (unsupported nu: not
+ *  an integer |nu| <= 200 or half-integer |nu| <= 19/2, or a negative
+ *  half-integer, which the reference answers with J_{|nu|}), -3..-6 (table
+ *  index), -100 (NULL options path), -501 (out of memory). */
 int nufht_plan_init(NufhtPlan *plan,
                     double nu,
                     const double *rs, int m,
