@@ -417,6 +417,12 @@ static void free_program(Program *P)
     hipFree(P->d_aimt);
     hipFree(P->d_cheb);
     hipFree(P->d_jtab);
+    if (P->side) {
+        hipStreamSynchronize(P->side);
+        hipStreamDestroy(P->side);
+    }
+    if (P->ev_fork) hipEventDestroy(P->ev_fork);
+    if (P->ev_join) hipEventDestroy(P->ev_join);
     for (auto &c : P->fft) {
         if (c.plan) rocfft_plan_destroy(c.plan);
         if (c.info) rocfft_execution_info_destroy(c.info);
@@ -1272,28 +1278,39 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         }
     }
     clk.mark("FFT tables + rocFFT plans");
+    // the second stream of the near-field stages (default priority)
+    if (!P->lproj.empty() || !P->dir.empty()) {
+        HIP_TRY(hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
+    }
     return 0;
 }
 
 // ---------------------------------------------------------------------------
 // program execution
+// The asymptotic stages in two parts: grid zeroing and spreading (part 0),
+// then the FFTs and interpolation (part 1).
 template <typename R, int EM>
-static void launch_asy(Program *P, hipStream_t st)
+static void launch_asy(Program *P, hipStream_t st, int part)
 {
-    asy_grid_zero<R>(P, st);
-    asy_spread<R, EM>(P, st);
-    asy_fft<R>(P, st);
-    asy_interp<R, EM>(P, st);
+    if (part == 0) {
+        asy_grid_zero<R>(P, st);
+        asy_spread<R, EM>(P, st);
+    } else {
+        asy_fft<R>(P, st);
+        asy_interp<R, EM>(P, st);
+    }
 }
 
 template <typename R>
-static void launch_asy_e(Program *P, hipStream_t st)
+static void launch_asy_e(Program *P, hipStream_t st, int part)
 {
     switch (P->emax) {
-    case 6: launch_asy<R, 6>(P, st); break;
-    case 10: launch_asy<R, 10>(P, st); break;
-    case 14: launch_asy<R, 14>(P, st); break;
-    default: launch_asy<R, 22>(P, st); break;
+    case 6: launch_asy<R, 6>(P, st, part); break;
+    case 10: launch_asy<R, 10>(P, st, part); break;
+    case 14: launch_asy<R, 14>(P, st, part); break;
+    default: launch_asy<R, 22>(P, st, part); break;
     }
 }
 
@@ -1323,21 +1340,18 @@ static void launch_loc_proj(Program *P, hipStream_t st, double *coef)
 
 // coef: the local-expansion moments' buffer (coef, or the caller's
 // scratch of at least P->ncoef doubles); it is cleared before use.
-static int run_program(Program *P, hipStream_t st, double *coef)
+// Where the near-field stages fork off: 0 never (one stream), 1 before the
+// spread (measured best: clustered step 12.04 -> 11.71 ms mixed, 15.73 ->
+// 15.37 FP64, headline 12.03 -> 11.93; forking before the FFTs gained about
+// half of that), 2 before the FFTs.  NUFHT_AMD_FORK overrides.
+static int g_fork_at = [] {
+    const char *e = getenv("NUFHT_AMD_FORK");
+    return e ? atoi(e) : 1;
+}();
+
+// Near-field stages: local projection and evaluation, then the direct sums.
+static int run_near_field(Program *P, hipStream_t st, double *coef)
 {
-    if (P->desc_dirty) {
-        HIP_TRY(hipMemcpyAsync(P->d_tfs, P->tfs.data(), P->tfs.size() * sizeof(TfDesc),
-                               hipMemcpyHostToDevice, st));
-        P->desc_dirty = false;
-    }
-    if (!P->zero.empty()) {
-        StageTimer tm(ST_ZERO, st);
-        hipLaunchKernelGGL(k_zero, dim3((unsigned)P->zero.size()), dim3(256), 0, st, P->d_tfs,
-                           P->d_zero);
-        if (P->ncoef)
-            hipLaunchKernelGGL(k_fill0, dim3((unsigned)std::min<long long>(1024, (P->ncoef + 255) / 256)),
-                               dim3(256), 0, st, coef, P->ncoef);
-    }
     if (!P->lproj.empty()) {
         {
             StageTimer tm(ST_LOCP, st);
@@ -1372,10 +1386,51 @@ static int run_program(Program *P, hipStream_t st, double *coef)
         hipLaunchKernelGGL(k_dir, dim3((unsigned)P->dir.size()), dim3(256), 0, st, P->d_tfs,
                            P->d_dir, P->d_dir_rng, P->d_jtab);
     }
-    if (!P->aspread.empty() || !P->adense.empty()) {
-        if (P->prec == 2) launch_asy_e<float>(P, st);
-        else launch_asy_e<double>(P, st);
+    return 0;
+}
+
+// coef: the local-expansion moments' buffer (P->d_coef, or the caller's
+// scratch of at least P->ncoef doubles); it is cleared before use.
+//
+// The near-field stages and the asymptotic stages only meet in their
+// additions into g (atomic, order-free), so when a program has both, the
+// near-field stages run on the program's second stream, forked from `st`
+// after g and the moments are zeroed and joined back before the call ends:
+// they fill the cycles the latency-bound spread, FFT and interpolation
+// kernels leave idle.  Inside a graph capture the fork and join are captured
+// as dependencies like any others.
+static int run_program(Program *P, hipStream_t st, double *coef)
+{
+    if (P->desc_dirty) {
+        HIP_TRY(hipMemcpyAsync(P->d_tfs, P->tfs.data(), P->tfs.size() * sizeof(TfDesc),
+                               hipMemcpyHostToDevice, st));
+        P->desc_dirty = false;
     }
+    if (!P->zero.empty()) {
+        StageTimer tm(ST_ZERO, st);
+        hipLaunchKernelGGL(k_zero, dim3((unsigned)P->zero.size()), dim3(256), 0, st, P->d_tfs,
+                           P->d_zero);
+        if (P->ncoef)
+            hipLaunchKernelGGL(k_fill0, dim3((unsigned)std::min<long long>(1024, (P->ncoef + 255) / 256)),
+                               dim3(256), 0, st, coef, P->ncoef);
+    }
+    const bool asy = !P->aspread.empty() || !P->adense.empty();
+    const bool near = !P->lproj.empty() || !P->dir.empty();
+    const bool fork = asy && near && P->side && g_fork_at > 0;
+    if (!fork) RC_TRY(run_near_field(P, st, coef));
+    for (int part = 0; part < 2; ++part) {
+        if (fork && part == g_fork_at - 1) {
+            HIP_TRY(hipEventRecord(P->ev_fork, st));
+            HIP_TRY(hipStreamWaitEvent(P->side, P->ev_fork, 0));
+            RC_TRY(run_near_field(P, P->side, coef));
+            HIP_TRY(hipEventRecord(P->ev_join, P->side));
+        }
+        if (asy) {
+            if (P->prec == 2) launch_asy_e<float>(P, st, part);
+            else launch_asy_e<double>(P, st, part);
+        }
+    }
+    if (fork) HIP_TRY(hipStreamWaitEvent(st, P->ev_join, 0));
     HIP_TRY(hipGetLastError());
     return 0;
 }

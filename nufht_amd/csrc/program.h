@@ -133,6 +133,9 @@ struct Program {
     long long jtab_total = 0;
     NufhtPlanStats stats;
     bool desc_dirty = true;
+    // second stream of the near-field stages and its fork / join events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 // Stage launchers of the asymptotic regime, instantiated per grid precision
