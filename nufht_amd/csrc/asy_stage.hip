@@ -17,6 +17,29 @@
 
 namespace nufht_dev {
 
+// The matrix-core spread's launches, one per products-per-wave count KW
+// (items [adense_off[KW], adense_off[KW + 1]) of the program's list).
+template <typename R, int EM, bool SP, int KW>
+static void spread_mfma_launch(Program *P, hipStream_t st)
+{
+    typedef typename Cplx<R>::T G;
+    if constexpr (KW >= 1) {
+        const int a = P->adense_off[KW], b = P->adense_off[KW + 1];
+        if (b > a) {
+            const size_t lds = SpreadSched<SP>::template lds<R, EM>();
+            static std::once_flag attr_once;
+            std::call_once(attr_once, [lds]() {
+                hipFuncSetAttribute((const void *)k_asy_spread_mfma<R, EM, SP, KW>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            });
+            hipLaunchKernelGGL((k_asy_spread_mfma<R, EM, SP, KW>), dim3((unsigned)(b - a)),
+                               dim3(SpreadSched<SP>::threads), lds, st, P->d_tfs, P->d_abox,
+                               P->d_adense + a, P->d_adrange + a, (G *)P->d_grid, P->d_ft);
+        }
+        spread_mfma_launch<R, EM, SP, KW - 1>(P, st);
+    }
+}
+
 template <typename R, int EM>
 void asy_spread(Program *P, hipStream_t st)
 {
@@ -29,25 +52,8 @@ void asy_spread(Program *P, hipStream_t st)
     }
     if (!P->adense.empty()) {
         StageTimer tm(ST_SPREAD_MF, st);
-        static std::once_flag attr_once;
-        std::call_once(attr_once, []() {
-            hipFuncSetAttribute((const void *)k_asy_spread_mfma<R, EM, true>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)SpreadSched<true>::template lds<R, EM>());
-            hipFuncSetAttribute((const void *)k_asy_spread_mfma<R, EM, false>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)SpreadSched<false>::template lds<R, EM>());
-        });
-        const size_t lds_split = SpreadSched<true>::template lds<R, EM>();
-        const size_t lds_unified = SpreadSched<false>::template lds<R, EM>();
-        if (P->spread_split)
-            hipLaunchKernelGGL((k_asy_spread_mfma<R, EM, true>), dim3((unsigned)P->adense.size()),
-                               dim3(SpreadSched<true>::threads), lds_split, st, P->d_tfs,
-                               P->d_abox, P->d_adense, P->d_adrange, (G *)P->d_grid, P->d_ft);
-        else
-            hipLaunchKernelGGL((k_asy_spread_mfma<R, EM, false>), dim3((unsigned)P->adense.size()),
-                               dim3(SpreadSched<false>::threads), lds_unified, st, P->d_tfs,
-                               P->d_abox, P->d_adense, P->d_adrange, (G *)P->d_grid, P->d_ft);
+        if (P->spread_split) spread_mfma_launch<R, EM, true, (2 * EM + 3) / 4>(P, st);
+        else spread_mfma_launch<R, EM, false, (2 * EM + 3) / 4>(P, st);
     }
 }
 

@@ -245,4 +245,12 @@ constexpr int kSP = 16;           // accumulator columns: a window of grid posit
 // them) and atomic adds elsewhere (zeroed beforehand).
 struct SpreadRange { int2 piece, wr, ex, pad; };
 
+// Products per consumer wave of an item: 2 per exponent group (real and
+// imaginary part), nsub = 16 / tb exponents per group, split over 4 waves.
+__host__ __device__ inline int spread_mfma_kw(int E, int tb)
+{
+    const int nsub = 16 / tb;
+    return (2 * ((E + nsub - 1) / nsub) + 3) >> 2;
+}
+
 }  // namespace nufht_dev

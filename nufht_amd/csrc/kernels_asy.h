@@ -537,24 +537,13 @@ template <bool SPLIT> struct SpreadSched {
     }
 };
 
+// One launch per products-per-wave count KW (the host groups the items by
+// spread_mfma_kw), so that each launch holds only the accumulators its items
+// use: with KW <= 5 and the split schedule at most 128 VGPRs, so that two
+// blocks share a CU (the few values that then leave registers are reloaded
+// once per chunk).
 template <typename R, int EMAX, bool SP, int KW>
-__device__ __forceinline__ void spread_mfma_dispatch(int kw, const TfDesc &T, const AsyBoxD &bx,
-                                                     const AsyItem &it, const SpreadRange &rg,
-                                                     int nprod,
-                                                     typename Cplx<R>::T *__restrict__ grid,
-                                                     const double *__restrict__ ft,
-                                                     SpreadLds<R, EMAX> *L)
-{
-    if constexpr (KW >= 1) {
-        if (kw == KW) spread_mfma_body<R, EMAX, KW, SP>(T, bx, it, rg, nprod, grid, ft, L);
-        else spread_mfma_dispatch<R, EMAX, SP, KW - 1>(kw, T, bx, it, rg, nprod, grid, ft, L);
-    }
-}
-
-// (split schedule, EMAX <= 10: at most 128 VGPRs, so that two blocks share a
-// CU; the few values that then leave registers are reloaded once per chunk)
-template <typename R, int EMAX, bool SP>
-__global__ __launch_bounds__(SpreadSched<SP>::threads, (SP && EMAX <= 10) ? 4 : 1) void k_asy_spread_mfma(
+__global__ __launch_bounds__(SpreadSched<SP>::threads, (SP && KW <= 5) ? 4 : 1) void k_asy_spread_mfma(
     const TfDesc *__restrict__ tfs, const AsyBoxD *__restrict__ boxes,
     const AsyItem *__restrict__ items, const SpreadRange *__restrict__ ranges,
     typename Cplx<R>::T *__restrict__ grid, const double *__restrict__ ft)
@@ -568,12 +557,8 @@ __global__ __launch_bounds__(SpreadSched<SP>::threads, (SP && EMAX <= 10) ? 4 : 
     const AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
     const int nsub = 16 / it.tb;
-    // products: 2 per exponent group; the per-wave count is uniform per
-    // block, one branch-free body per count
     const int nprod = 2 * ((bx.E + nsub - 1) / nsub);
-    const int kw = (nprod + 3) >> 2;
-    constexpr int KWMAX = (2 * EMAX + 3) / 4;
-    spread_mfma_dispatch<R, EMAX, SP, KWMAX>(kw, T, bx, it, rg, nprod, grid, ft, L);
+    spread_mfma_body<R, EMAX, KW, SP>(T, bx, it, rg, nprod, grid, ft, L);
 }
 
 

@@ -117,6 +117,7 @@ struct Program {
     std::vector<AsyItem> aimt;        // matrix-core interpolation tiles (knob interp_mfma 2)
     std::vector<int2> arange;
     std::vector<SpreadRange> adrange;
+    std::vector<int> adense_off;      // matrix-core spread items by products per wave
     AsyItem *d_aspread = nullptr, *d_ainterp = nullptr, *d_azero = nullptr, *d_adense = nullptr;
     InterpRun *d_aimf = nullptr;
     AsyItem *d_aimt = nullptr;
