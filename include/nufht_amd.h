@@ -149,6 +149,10 @@ typedef struct {
     double model_f64, model_f32, model_bytes;
     /* the direct-sum and local-expansion shares of model_f64 */
     double model_dir, model_loc;
+    /* asymptotic boxes run by the fused small-box kernel, and their
+     * (source, channel) and (target, channel) pairs (not in the spread_sc_*
+     * and interp_tc_* counts) */
+    long long fused_boxes, fused_sc, fused_tc;
 } NufhtPlanStats;
 int nufht_amd_plan_stats(const NufhtPlan *plan, int batch, NufhtPlanStats *st);
 /* The same statistics summed over the transforms of a multi-transform batch
@@ -171,6 +175,13 @@ int nufht_multi_stats(const NufhtMulti *mt, NufhtPlanStats *st);
  *   spread_sched  matrix-core spread schedule: 1 split (4 waves stage source
  *                 chunks into double-buffered LDS while 4 multiply), 0 unified
  *                 (4 waves stage, then multiply), -1 automatic (split)
+ *   fused_max     asymptotic boxes with a uniform grid of at most 256 points
+ *                 and at most fused_max sources and targets run the fused
+ *                 kernel: spread, FFT and interpolation with the grid in LDS,
+ *                 in one persistent launch over a work queue of their units
+ *                 (default 0: every box runs the staged kernels, measured
+ *                 faster on the benchmark workloads; NUFHT_AMD_FUSED_MAX
+ *                 overrides the default)
  *   precision     one of NUFHT_PREC_* (as nufht_amd_plan_set_precision)
  * A plan starts from the defaults (NUFHT_AMD_DENSE_CELL and
  * NUFHT_AMD_PRECISION override two of them). */
@@ -180,13 +191,14 @@ typedef struct {
     int interp_mfma, own_fft;
     int precision;
     int spread_sched;
+    int fused_max;
 } NufhtAmdKnobs;
 int nufht_amd_get_knobs(const NufhtPlan *plan, NufhtAmdKnobs *k);
 int nufht_amd_set_knobs(NufhtPlan *plan, const NufhtAmdKnobs *k);
 
 /* Times the knobs on the device with the plan's own geometry and batch
  * (coordinate descent: precision when tol >= 1e-6, then dense_cell,
- * interp_mfma, dense_split, spread_split, own_fft, spread_sched; each candidate is built,
+ * interp_mfma, dense_split, spread_split, own_fft, spread_sched, fused_max; each candidate is built,
  * run once untimed and timed over three calls) and keeps the fastest. */
 int nufht_amd_autotune(NufhtPlan *plan, int batch, void *stream);
 /* The same for a multi-transform batch (precision stays per plan); the

@@ -84,7 +84,8 @@ class PlanStats(ctypes.Structure):
         [(n, ctypes.c_longlong) for n in (
             "spread_sc_valu", "spread_sc_mfma", "interp_tc_valu", "interp_tc_mfma")] + \
         [(n, ctypes.c_double) for n in ("asy_fine", "asy_fine_nlogn", "fft_bytes", "model_f64",
-                                        "model_f32", "model_bytes", "model_dir", "model_loc")]
+                                        "model_f32", "model_bytes", "model_dir", "model_loc")] + \
+        [(n, ctypes.c_longlong) for n in ("fused_boxes", "fused_sc", "fused_tc")]
 
 
 class Knobs(ctypes.Structure):
@@ -92,7 +93,7 @@ class Knobs(ctypes.Structure):
     _fields_ = [("dense_cell", ctypes.c_double), ("dense_split", ctypes.c_int),
                 ("spread_split", ctypes.c_int), ("interp_mfma", ctypes.c_int),
                 ("own_fft", ctypes.c_int), ("precision", ctypes.c_int),
-                ("spread_sched", ctypes.c_int)]
+                ("spread_sched", ctypes.c_int), ("fused_max", ctypes.c_int)]
 
 
 def _knobs_dict(k: Knobs) -> Dict[str, float]:
@@ -486,7 +487,7 @@ class MultiBatch:
 # per-stage device timing (HIP events around each stage launch)
 STAGES = ["zero", "permute", "direct", "loc_project", "loc_eval", "asy_grid_zero",
           "asy_spread", "asy_spread_mfma", "asy_fft", "asy_interp", "asy_interp_mfma",
-          "grad_combine"]
+          "grad_combine", "asy_fused"]
 
 
 def profile(enable: bool = True):

@@ -4,6 +4,7 @@
 // (each pair is its own object, so that the kernels' instantiations compile
 // in parallel).
 #include "kernels_asy.h"
+#include "kernels_fused.h"
 #include "program.h"
 
 #include <mutex>
@@ -84,6 +85,31 @@ void asy_interp(Program *P, hipStream_t st)
     }
 }
 
+// (internal linkage: every exponent-bound object defines its own copy)
+static __global__ void k_queue_reset(int *q) { *q = 0; }
+
+// The fused small boxes: one persistent launch over the unit queue (its
+// counter cleared by a kernel first, so that the pair can be captured in a
+// graph and replayed).
+template <typename R, int EM>
+void asy_fused(Program *P, hipStream_t st)
+{
+    typedef typename Cplx<R>::T G;
+    if (P->funits.empty()) return;
+    StageTimer tm(ST_FUSED, st);
+    const size_t lds = (size_t)P->fused_grid_elems * sizeof(G) + fused_stage_bytes<R, EM>();
+    static std::once_flag attr_once;
+    std::call_once(attr_once, []() {
+        hipFuncSetAttribute((const void *)k_asy_fused<R, EM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kFusedGridBytes + fused_stage_bytes<R, EM>());
+    });
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, st, P->d_fqueue);
+    hipLaunchKernelGGL((k_asy_fused<R, EM>), dim3((unsigned)P->fused_blocks), dim3(kFusedThreads), lds,
+                       st, P->d_tfs, P->d_abox, P->d_funits, P->d_fpass, (int)P->funits.size(),
+                       P->d_fqueue, (const G *)P->d_fftw, P->d_ft, P->fused_grid_elems);
+}
+
+template void asy_fused<NUFHT_ASY_R, NUFHT_ASY_EM>(Program *, hipStream_t);
 template void asy_spread<NUFHT_ASY_R, NUFHT_ASY_EM>(Program *, hipStream_t);
 template void asy_interp<NUFHT_ASY_R, NUFHT_ASY_EM>(Program *, hipStream_t);
 

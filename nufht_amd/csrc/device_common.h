@@ -85,11 +85,22 @@ __device__ __forceinline__ double ft_eval(const double *c, double k, double kmax
 // three parts, the first two with 30 significant bits so that n P1 and n P2
 // are exact for |n| < 2^22), Taylor polynomials of sin and cos on
 // |r| <= pi/4 (coefficients (-1)^k / j!, truncation below 1e-19), and the
-// quadrant of n.  Arguments beyond 2^21 go to the library routine.
+// quadrant of n.  Arguments beyond 2^21 go to the library routine: inlined
+// (SC_INLINE), out of line (SC_CALL: its full-range reduction holds about 40
+// registers wherever it is inlined, although only |v| >= 2^21 reaches it), or
+// not at all (SC_REDUCED: the caller guarantees |v| < 2^21).
+enum { SC_INLINE, SC_CALL, SC_REDUCED };
+__device__ __attribute__((noinline)) inline void sincos_full(double v, double *s, double *c)
+{
+    sincos(v, s, c);
+}
+
+template <int MODE = SC_INLINE>
 __device__ __forceinline__ void sincos_cw(double v, double *s, double *c)
 {
-    if (!(fabs(v) < 2097152.0)) {
-        sincos(v, s, c);
+    if (MODE != SC_REDUCED && !(fabs(v) < 2097152.0)) {
+        if (MODE == SC_CALL) sincos_full(v, s, c);
+        else sincos(v, s, c);
         return;
     }
     const double P1 = 0x1.921fb54p+0, P2 = 0x1.10b46118p-30, P3 = 0x1.313198a2e0370p-61;
@@ -175,7 +186,18 @@ __device__ __forceinline__ float es_kernel_f(float u, float beta)
     return t > 0.0f ? __expf(beta * (__fsqrt_rn(t) - 1.0f)) : 0.0f;
 }
 
-constexpr int kSpreadQ = 128;      // sources staged per LDS pass
+constexpr int kVQ = 64;            // sources staged per LDS pass of the VALU spread
+// Exponents per lane of the VALU spread (4 EG accumulators: two columns,
+// complex): channels are split over lanes, so a lane's registers do not grow
+// with the box's E.
+template <typename R> struct SpreadEG { static constexpr int v = 4; };
+template <> struct SpreadEG<float> { static constexpr int v = 8; };
+// lanes per grid position of a VALU spread item: column pairs x exponent groups
+__host__ __device__ inline int spread_valu_lanes(int E, int tb, bool f32)
+{
+    const int eg = f32 ? 8 : 4;
+    return (tb > 1 ? tb >> 1 : 1) * ((E + eg - 1) / eg);
+}
 constexpr int kTapMax = 17;        // kernel width + 1
 constexpr int kInterpWin = 64;     // grid positions staged per target tile
 

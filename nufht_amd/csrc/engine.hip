@@ -81,7 +81,7 @@ struct PhaseClock {
 static const char *kStageNames[ST_COUNT] = {"zero", "permute", "direct", "loc_project",
                                             "loc_eval", "asy_grid_zero", "asy_spread",
                                             "asy_spread_mfma", "asy_fft", "asy_interp",
-                                            "asy_interp_mfma", "grad_combine"};
+                                            "asy_interp_mfma", "grad_combine", "asy_fused"};
 struct PendingEv { hipEvent_t a, b; int stage; };
 static std::mutex g_prof_mu;
 static bool g_prof_on = false;
@@ -252,6 +252,12 @@ static double g_dense_cell = [] {
     return e ? atof(e) : kDenseCell;
 }();
 double nufht_dev::knobs_default_dense_cell() { return g_dense_cell; }
+// (NUFHT_AMD_FUSED_MAX overrides the fused kernel's box size limit)
+static int g_fused_max = [] {
+    const char *e = getenv("NUFHT_AMD_FUSED_MAX");
+    return e ? std::max(0, atoi(e)) : kFusedMax;
+}();
+int nufht_dev::knobs_default_fused_max() { return g_fused_max; }
 
 
 // Two-pass factorisation Nf = N1 N2 for the FFT kernels (Nf = 2^a, 3 2^a or
@@ -414,6 +420,7 @@ static void free_program(Program *P)
     hipFree(P->d_aspread); hipFree(P->d_ainterp); hipFree(P->d_grid); hipFree(P->d_ft);
     hipFree(P->d_grid2); hipFree(P->d_fftw); hipFree(P->d_fft_chinfo);
     hipFree(P->d_azero); hipFree(P->d_arange); hipFree(P->d_adense); hipFree(P->d_adrange);
+    hipFree(P->d_funits); hipFree(P->d_fpass); hipFree(P->d_fqueue);
     hipFree(P->d_aimf);
     hipFree(P->d_aimt);
     hipFree(P->d_cheb);
@@ -727,6 +734,7 @@ static void add_dir_items(Program *P, int tf, int B, const std::vector<Box> &dir
 static void add_interp_items(Program *P, const std::vector<TfInput> &in)
 {
     for (size_t bi = 0; bi < P->abox.size(); ++bi) {
+        if (P->afused[bi]) continue;
         const int box = (int)bi;
         const AsyBoxD &A = P->abox[bi];
         const TfDesc &T = P->tfs[A.tf];
@@ -793,6 +801,81 @@ constexpr int kLocSlotsMax = 128;
 static bool loc_fits(int order, int half, int K)
 {
     return K >= 0 && (K + 16) / 16 <= kLocMaxNlt && loc_slots(order, half, K) <= kLocSlotsMax;
+}
+
+// Whether asymptotic box A runs in the fused kernel: a uniform grid of at
+// most kFusedNfMax points, at most fused_max sources and targets (one block
+// takes each of its units), a kernel width the tap staging holds, and phases
+// D x and w C below 2^20, which the kernel reduces without the library's
+// full-range path.
+static bool fused_fits(const Program *P, const AsyBoxD &A, const TfDesc &T)
+{
+    if (P->knobs.fused_max <= 0 || A.Nf > kFusedNfMax || A.nj > P->knobs.fused_max ||
+        A.ni > P->knobs.fused_max || T.kw + 1 > kTapMax)
+        return false;
+    const double X = (A.L + 1) * A.h, S = M_PI / A.h;          // bounds of |x| and |s|
+    return std::fabs(A.D) * X < 1048576.0 && (std::fabs(A.D) + S) * std::fabs(A.C) < 1048576.0;
+}
+
+// LDS layout of a fused unit's grid (Nf points x M channels): padded rows,
+// or compact ones where the padding would pass the grid budget.
+static void fused_layout(FftPass &p, size_t esz)
+{
+    p.lds_stride = p.M == 1 ? 1 : p.M + 1;
+    p.lds_shift = 3;
+    if (fft_lds_elems(p.N, p.lds_stride, p.lds_shift, p.M) * esz > (size_t)kFusedGridBytes) {
+        p.lds_stride = p.M;
+        p.lds_shift = 2;
+    }
+}
+
+// Units of fused box `box` (column chunk tb): as many channels per unit as
+// the block's lanes (2L + 1 positions x column pairs x exponent groups <=
+// kFusedThreads) and the LDS grid budget allow.
+static void add_fused_units(Program *P, int box, int tb, bool mixed,
+                            std::map<std::pair<int, int>, int> &fpass_of)
+{
+    const AsyBoxD &A = P->abox[box];
+    const TfDesc &T = P->tfs[A.tf];
+    const size_t esz = mixed ? sizeof(float2) : sizeof(double2);
+    const int lch = fused_lane_ch(mixed), Pn = 2 * A.L + 1;
+    auto ntl_of = [](int t) { return t >= 2 ? t / 2 : 1; };
+    int tbu = tb, ne = 0, M = 1;
+    for (; tbu >= 1; tbu >>= 1) {
+        if (Pn * ntl_of(tbu) > kFusedThreads) continue;
+        const int eb = tbu >= 2 ? lch / 2 : lch;
+        ne = std::min(A.E, (kFusedThreads / (Pn * ntl_of(tbu))) * eb);
+        for (; ne >= 1; --ne) {
+            M = 1;
+            while (M < ne * tbu) M <<= 1;
+            FftPass p;
+            memset(&p, 0, sizeof(p));
+            p.N = A.Nf;
+            p.M = M;
+            fused_layout(p, esz);
+            if (fft_lds_elems(p.N, p.lds_stride, p.lds_shift, p.M) * esz <= (size_t)kFusedGridBytes) break;
+        }
+        if (ne >= 1) break;
+    }
+    auto key = std::make_pair(A.Nf, M);
+    if (!fpass_of.count(key)) {
+        FftPass p;
+        memset(&p, 0, sizeof(p));
+        p.N = A.Nf;
+        p.nf = A.Nf;
+        p.M = M;
+        while ((1 << p.lgM) < M) ++p.lgM;
+        fft_radices(A.Nf, p);
+        fused_layout(p, esz);
+        fpass_of[key] = (int)P->fpass.size();
+        P->fpass.push_back(p);
+        P->fused_grid_elems = std::max(P->fused_grid_elems,
+                                       (int)fft_lds_elems(p.N, p.lds_stride, p.lds_shift, p.M));
+    }
+    const int fp = fpass_of[key];
+    for (int c0 = 0; c0 < T.B; c0 += tbu)
+        for (int e0 = 0; e0 < A.E; e0 += ne)
+            P->funits.push_back({box, c0, e0, std::min(ne, A.E - e0), tbu, fp});
 }
 
 static int build_program(Program *P, const std::vector<TfInput> &in)
@@ -955,9 +1038,14 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         }
         P->tfs.push_back(T);
     }
-    // asymptotic boxes grouped by FFT length; classes laid out back to back
+    // small boxes whose sources and targets fit the fused kernel's units run
+    // there (no HBM grid, no FFT class)
+    std::vector<char> fz(apend.size(), 0);
+    for (size_t q = 0; q < apend.size(); ++q) fz[q] = fused_fits(P, apend[q].b, P->tfs[apend[q].b.tf]);
+    // the other asymptotic boxes grouped by FFT length; classes laid out back to back
     std::map<int, std::vector<int>> byNf;
-    for (size_t q = 0; q < apend.size(); ++q) byNf[apend[q].b.Nf].push_back((int)q);
+    for (size_t q = 0; q < apend.size(); ++q)
+        if (!fz[q]) byNf[apend[q].b.Nf].push_back((int)q);
     long long off = 0;
     std::map<int, bool> own_nf;
     for (auto &kv : byNf) {
@@ -1023,14 +1111,27 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         }
         dense_split = kwd <= 8 ? 512 : 1024;
     }
-    for (auto &ap : apend) {
+    std::map<std::pair<int, int>, int> fpass_of;
+    for (size_t q = 0; q < apend.size(); ++q) {
+        const AsyPending &ap = apend[q];
         int box = (int)P->abox.size();
         const AsyBoxD &A = ap.b;
         const TfDesc &T = P->tfs[A.tf];
         const NufhtImplHost *h = (const NufhtImplHost *)in[A.tf].plan->impl;
         P->abox.push_back(A);
+        P->afused.push_back(fz[q]);
+        if (fz[q]) {
+            add_fused_units(P, box, ap.tb, mixed, fpass_of);
+            P->stats.fused_boxes++;
+            P->stats.fused_sc += (long long)A.nj * A.E * T.B;
+            P->stats.fused_tc += (long long)A.ni * A.E * T.B;
+            P->stats.asy_grid += A.Nf;
+            P->stats.asy_grid_points += (long long)A.E * T.B * A.Nf;
+            continue;
+        }
         int tb = ap.tb;
-        int sp = 256 / std::max(1, tb / 2);            // spread positions per item
+        // spread positions per VALU item: 256 lanes over column pairs x exponent groups
+        int sp = std::max(1, 256 / spread_valu_lanes(A.E, tb, mixed));
         const double *rb = h->r + A.j0, *re = h->r + A.j0 + A.nj;
         // source-dense boxes (many sources per grid cell) use the split-K kernel
         // (the matrix-core spread slides a 16-position window: kw <= 13 keeps
@@ -1117,6 +1218,25 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         P->stats.asy_grid_points += (long long)A.E * T.B * A.Nf;
     }
     add_interp_items(P, in);
+    // fused units in decreasing cost (the queue hands out large units first)
+    if (!P->funits.empty()) {
+        std::vector<double> cost(P->funits.size());
+        std::vector<size_t> order(cost.size());
+        for (size_t q = 0; q < cost.size(); ++q) {
+            const FusedUnit &u = P->funits[q];
+            const AsyBoxD &A = P->abox[u.box];
+            const int kw = P->tfs[A.tf].kw;
+            const double ch = (double)u.ne * u.tb;
+            cost[q] = ch * ((kw + 1.0) * A.nj + 1.6 * kw * A.ni + 5.0 * A.Nf * std::log2((double)A.Nf)) +
+                      400.0 * (A.nj + A.ni);
+            order[q] = q;
+        }
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+        std::vector<FusedUnit> u2;
+        u2.reserve(order.size());
+        for (size_t q : order) u2.push_back(P->funits[q]);
+        P->funits.swap(u2);
+    }
     P->stats.fft_classes = (long long)P->fft.size();
     P->emax = e_max <= 6 ? 6 : e_max <= 10 ? 10 : e_max <= 14 ? 14 : 22;
     // matrix-core spread items grouped (stably: box order kept within a
@@ -1208,6 +1328,14 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     RC_TRY(upload(&P->d_aimf, P->aimf));
     RC_TRY(upload(&P->d_aimt, P->aimt));
     RC_TRY(upload(&P->d_adrange, P->adrange));
+    RC_TRY(upload(&P->d_funits, P->funits));
+    if (!P->funits.empty()) {
+        HIP_TRY(hipMalloc((void **)&P->d_fqueue, sizeof(int)));
+        int dev = 0, ncu = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        P->fused_blocks = (int)std::min<size_t>(P->funits.size(), (size_t)std::max(1, 2 * ncu));
+    }
     for (auto &it : P->ainterp) P->interp_tmax = std::max(P->interp_tmax, 256 / it.tb);
     RC_TRY(upload(&P->d_ft, ft_all));
     if (cheb_all.empty()) cheb_all.push_back(0.0);
@@ -1267,6 +1395,18 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         c.lds1 = fft_lds_elems(c.p1.N, c.p1.lds_stride, c.p1.lds_shift, c.p1.M) * esz;
         c.lds2 = fft_lds_elems(c.p2.N, c.p2.lds_stride, c.p2.lds_shift, c.p2.M) * esz;
     }
+    {
+        std::map<int, int> w_of;
+        for (auto &p : P->fpass) {
+            if (!w_of.count(p.N)) {
+                int off;
+                fft_table(tw, p.N, p.N, &off);
+                w_of[p.N] = off;
+            }
+            p.w_off = w_of[p.N];
+        }
+        RC_TRY(upload(&P->d_fpass, P->fpass));
+    }
     if (!P->fft_chinfo.empty()) RC_TRY(upload(&P->d_fft_chinfo, P->fft_chinfo));
     if (!tw.empty()) {
         if (mixed) {
@@ -1314,12 +1454,13 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
 
 // ---------------------------------------------------------------------------
 // program execution
-// The asymptotic stages in two parts: grid zeroing and spreading (part 0),
-// then the FFTs and interpolation (part 1).
+// The asymptotic stages in two parts: the fused small boxes, grid zeroing and
+// spreading (part 0), then the FFTs and interpolation (part 1).
 template <typename R, int EM>
 static void launch_asy(Program *P, hipStream_t st, int part)
 {
     if (part == 0) {
+        asy_fused<R, EM>(P, st);
         asy_grid_zero<R>(P, st);
         asy_spread<R, EM>(P, st);
     } else {
@@ -1439,7 +1580,7 @@ static int run_program(Program *P, hipStream_t st, double *coef)
             hipLaunchKernelGGL(k_fill0, dim3((unsigned)std::min<long long>(1024, (P->ncoef + 255) / 256)),
                                dim3(256), 0, st, coef, P->ncoef);
     }
-    const bool asy = !P->aspread.empty() || !P->adense.empty();
+    const bool asy = !P->aspread.empty() || !P->adense.empty() || !P->funits.empty();
     const bool near = !P->lproj.empty() || !P->dir.empty();
     const bool fork = asy && near && P->side && g_fork_at > 0;
     if (!fork) RC_TRY(run_near_field(P, st, coef));
@@ -2026,6 +2167,7 @@ static void knobs_to_c(const Knobs &k, int prec, NufhtAmdKnobs *o)
     o->interp_mfma = k.interp_mfma;
     o->own_fft = k.own_fft;
     o->spread_sched = k.spread_sched;
+    o->fused_max = k.fused_max;
     o->precision = prec;
 }
 
@@ -2041,7 +2183,7 @@ static int knobs_check(const NufhtAmdKnobs *k)
     if (!(k->dense_cell > 0.0) || (k->dense_split != 0 && k->dense_split < kSQ) ||
         k->spread_split < 64 ||
         k->interp_mfma < 0 || k->interp_mfma > 2 || (k->own_fft != 0 && k->own_fft != 1) ||
-        k->spread_sched < -1 || k->spread_sched > 1 ||
+        k->spread_sched < -1 || k->spread_sched > 1 || k->fused_max < 0 ||
         (k->precision != NUFHT_PREC_FP64 && k->precision != NUFHT_PREC_AUTO &&
          k->precision != NUFHT_PREC_MIXED)) {
         nufht_set_error("invalid knobs");
@@ -2061,6 +2203,7 @@ extern "C" int nufht_amd_set_knobs(NufhtPlan *plan, const NufhtAmdKnobs *k)
     E->knobs.interp_mfma = k->interp_mfma;
     E->knobs.own_fft = k->own_fft;
     E->knobs.spread_sched = k->spread_sched;
+    E->knobs.fused_max = k->fused_max;
     ((NufhtImplHost *)plan->impl)->precision = k->precision;
     return 0;
 }
@@ -2076,6 +2219,7 @@ extern "C" int nufht_multi_get_knobs(const NufhtMulti *mt, NufhtAmdKnobs *k)
 static const double kTuneDense[] = {2.0, 4.0, 8.0, 1e300};
 static const int kTuneDenseSplit[] = {512, 1024, 2048, 4096};
 static const int kTuneSpreadSplit[] = {1024, 2048, 4096};
+static const int kTuneFusedMax[] = {0, 128, 512};
 
 // Times `run` (which enqueues one call on st): one untimed call (builds the
 // program), then the mean of three.  Returns a negative code on failure.
@@ -2126,6 +2270,7 @@ static int tune_knobs(Knobs &cur, Apply apply, Run run, hipStream_t st)
     for (int v : kTuneSpreadSplit) { Knobs c = cur; c.spread_split = v; consider(c); }
     for (int v = 1; v >= 0; --v) { Knobs c = cur; c.own_fft = v; consider(c); }
     for (int v = -1; v <= 1; ++v) { Knobs c = cur; c.spread_sched = v; consider(c); }
+    for (int v : kTuneFusedMax) { Knobs c = cur; c.fused_max = v; consider(c); }
     apply(cur);
     return 0;
 }

@@ -49,10 +49,13 @@ __global__ void k_grid_zero(const TfDesc *__restrict__ tfs, const AsyBoxD *__res
 // deconvolution factor are FP64 always.  An item owns a tile of grid
 // positions, a column chunk (tb columns) and a source sub-range [jlo, jhi);
 // items that split a tile's sources (pad == 1) add atomically into a zeroed
-// tile.  Each thread owns one position and two columns; the tb/2 lanes of a
-// position read the same tap table entries (LDS broadcast).
+// tile.  A lane owns one position, two columns and EG consecutive exponents
+// (spread_valu_lanes gives the lanes per position; the host sizes the tile
+// so that they fit in 256): the lanes of an exponent group and column pair
+// take consecutive positions, so that every store writes runs of adjacent
+// grid points, and a lane holds 4 EG accumulators whatever the box's E.
 template <typename R, int EMAX>
-__global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ tfs,
+__global__ __launch_bounds__(256, 4) void k_asy_spread(const TfDesc *__restrict__ tfs,
                                                     const AsyBoxD *__restrict__ boxes,
                                                     const AsyItem *__restrict__ items,
                                                     const int2 *__restrict__ ranges,
@@ -60,63 +63,75 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
                                                     const double *__restrict__ ft)
 {
     typedef typename Cplx<R>::T C2;
-    __shared__ C2 s_z[kSpreadQ][kTapMax];        // psi * e^{iDx} r^{-1/2}
-    __shared__ C2 s_c[kSpreadQ][8];              // column pairs
-    __shared__ double s_x[kSpreadQ], s_br[kSpreadQ], s_bi[kSpreadQ];
-    __shared__ R s_ri[kSpreadQ];
-    __shared__ int s_pf[kSpreadQ];
-    AsyItem it = items[blockIdx.x];
-    int2 jr = ranges[blockIdx.x];
-    AsyBoxD bx = boxes[it.box];
+    constexpr int EG = SpreadEG<R>::v;
+    constexpr int NG = (EMAX + EG - 1) / EG;
+    __shared__ C2 s_z[kVQ][kTapMax];       // psi * e^{iDx} r^{-1/2}, tap k of source q
+    __shared__ C2 s_c[8][kVQ];             // column pair, source
+    __shared__ R s_pw[kVQ][NG + 1];        // (rho/r)^(g EG) for group g; slot NG: rho/r
+    __shared__ double s_x[kVQ];
+    __shared__ double2 s_ph[kVQ];          // e^{iDx} r^{-1/2}
+    __shared__ int s_pf[kVQ];
+    __shared__ double s_dec[256];
+    const AsyItem it = items[blockIdx.x];
+    const int2 jr = ranges[blockIdx.x];
+    const AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
-    int tb = it.tb;
-    int tl2 = tb > 1 ? tb >> 1 : 1;          // lanes per position
-    int tl = threadIdx.x % tl2, pl = threadIdx.x / tl2;
-    int p = it.start + pl;
-    int l = p <= bx.L ? p : p - bx.Nf;
-    bool valid = (pl < it.count) && (l >= -bx.L) && (l <= bx.L);
-    int t0 = it.col0 + 2 * tl;
-    bool active = valid && (t0 < T.B);
-    int kw = T.kw;
-    double inv_a1 = 1.0 / bx.a1;
-    R a0r[EMAX], a0i[EMAX], a1r[EMAX], a1i[EMAX];
+    const int tb = it.tb, tl2 = tb > 1 ? tb >> 1 : 1;
+    const int E = bx.E, ng = (E + EG - 1) / EG;
+    const int np = it.count;
+    const int tid = threadIdx.x;
+    const int pl = tid % np, combo = tid / np;
+    const int tl = combo % tl2, g = combo / tl2;
+    const bool lane_on = g < ng;
+    const int p = it.start + pl;
+    const int l = p <= bx.L ? p : p - bx.Nf;
+    const bool valid = (l >= -bx.L) && (l <= bx.L);
+    const int t0 = it.col0 + 2 * tl;
+    const bool active = lane_on && valid && (t0 < T.B);
+    const int kw = T.kw;
+    const double inv_a1 = 1.0 / bx.a1;
+    R a0r[EG], a0i[EG], a1r[EG], a1i[EG];
 #pragma unroll
-    for (int e = 0; e < EMAX; ++e) { a0r[e] = a0i[e] = a1r[e] = a1i[e] = (R)0; }
-    int E = bx.E;
-    for (int c0 = jr.x; c0 < jr.y; c0 += kSpreadQ) {
-        int cn = min(kSpreadQ, jr.y - c0);
-        for (int q = threadIdx.x; q < cn; q += 256) {
-            double rj = T.r[c0 + q];
-            double x = rj - bx.C;
+    for (int e = 0; e < EG; ++e) { a0r[e] = a0i[e] = a1r[e] = a1i[e] = (R)0; }
+    for (int c0 = jr.x; c0 < jr.y; c0 += kVQ) {
+        const int cn = min(kVQ, jr.y - c0);
+        if (tid < cn) {
+            const double rj = T.r[c0 + tid];
+            const double x = rj - bx.C;
+            s_x[tid] = x;
+            s_pf[tid] = (int)ceil((x - bx.a1) / bx.h);
             double sn, cs;
-            sincos_cw(bx.D * x, &sn, &cs);
-            double rs = 1.0 / sqrt(rj);
-            int lf = (int)ceil((x - bx.a1) / bx.h);
-            s_x[q] = x;
-            s_br[q] = cs * rs;
-            s_bi[q] = sn * rs;
-            s_pf[q] = lf;
-            s_ri[q] = (R)(bx.rho / rj);
+            sincos_cw<SC_CALL>(bx.D * x, &sn, &cs);
+            const double rs = 1.0 / sqrt(rj);
+            s_ph[tid] = make_double2(cs * rs, sn * rs);
+            const double ri = bx.rho / rj;
+            double rg = 1.0;
+#pragma unroll
+            for (int e = 0; e < EG; ++e) rg *= ri;
+            double pw = 1.0;
+#pragma unroll
+            for (int gg = 0; gg < NG; ++gg) { s_pw[tid][gg] = (R)pw; pw *= rg; }
+            s_pw[tid][NG] = (R)ri;
         }
-        for (int q = threadIdx.x; q < cn * 8; q += 256) {
-            int jq = q >> 3, pr = q & 7;
-            int ta = it.col0 + 2 * pr, tb2 = ta + 1;
+        // coefficients: consecutive lanes load consecutive sources of a column
+        for (int q = tid; q < tl2 * cn; q += 256) {
+            const int pr = q / cn, jq = q - pr * cn;
+            const int ta = it.col0 + 2 * pr, tb2 = ta + 1;
             C2 v;
             v.x = ta < T.B && 2 * pr < tb ? (R)T.cs[(long long)ta * T.ld_cs + c0 + jq] : (R)0;
             v.y = tb2 < T.B && 2 * pr + 1 < tb ? (R)T.cs[(long long)tb2 * T.ld_cs + c0 + jq] : (R)0;
-            s_c[jq][pr] = v;
+            s_c[pr][jq] = v;
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < cn * kTapMax; q += 256) {
-            int jq = q / kTapMax, k = q % kTapMax;
-            double psi = 0.0;
-            if (k <= kw) {
-                double u = ((double)(s_pf[jq] + k) * bx.h - s_x[jq]) * inv_a1;
-                psi = (double)es_eval((R)u, (R)T.beta);
-            }
+        for (int q = tid; q < cn * (kw + 1); q += 256) {
+            const int jq = q / (kw + 1), k = q - jq * (kw + 1);
+            const double x = s_x[jq];
+            const double u = ((double)(s_pf[jq] + k) * bx.h - x) * inv_a1;
+            const double psi = (double)es_eval((R)u, (R)T.beta);
+            const double2 ph = s_ph[jq];
             C2 z;
-            z.x = (R)(psi * s_br[jq]);
-            z.y = (R)(psi * s_bi[jq]);
+            z.x = (R)(ph.x * psi);
+            z.y = (R)(ph.y * psi);
             s_z[jq][k] = z;
         }
         __syncthreads();
@@ -124,38 +139,48 @@ __global__ __launch_bounds__(256) void k_asy_spread(const TfDesc *__restrict__ t
             // sources of this chunk whose taps reach l: lf in [l - kw, l] (lf ascending)
             int a = 0, b = cn;
             while (a < b) { int mid = (a + b) >> 1; if (s_pf[mid] >= l - kw) b = mid; else a = mid + 1; }
-            int qa = a;
+            const int qa = a;
             b = cn;
             while (a < b) { int mid = (a + b) >> 1; if (s_pf[mid] > l) b = mid; else a = mid + 1; }
-            int qb = a;
+            const int qb = a;
             for (int q = qa; q < qb; ++q) {
-                C2 z = s_z[q][l - s_pf[q]];
-                C2 cc = s_c[q][tl];
-                R ri = s_ri[q];
-                R z0r = z.x * cc.x, z0i = z.y * cc.x, z1r = z.x * cc.y, z1i = z.y * cc.y;
+                const C2 z = s_z[q][l - s_pf[q]];
+                const C2 cc = s_c[tl][q];
+                const R w = s_pw[q][g], ri = s_pw[q][NG];
+                const R zr = z.x * w, zi = z.y * w;
+                const R z0r = zr * cc.x, z0i = zi * cc.x, z1r = zr * cc.y, z1i = zi * cc.y;
                 R rp = (R)1;
 #pragma unroll
-                for (int e = 0; e < EMAX; ++e) {
-                    if (e < E) {
-                        a0r[e] = fma(z0r, rp, a0r[e]);
-                        a0i[e] = fma(z0i, rp, a0i[e]);
-                        a1r[e] = fma(z1r, rp, a1r[e]);
-                        a1i[e] = fma(z1i, rp, a1i[e]);
-                        rp *= ri;
-                    }
+                for (int e = 0; e < EG; ++e) {
+                    a0r[e] = fma(z0r, rp, a0r[e]);
+                    a0i[e] = fma(z0i, rp, a0i[e]);
+                    a1r[e] = fma(z1r, rp, a1r[e]);
+                    a1i[e] = fma(z1i, rp, a1i[e]);
+                    rp *= ri;
                 }
             }
         }
         __syncthreads();
     }
-    if (pl < it.count && t0 < T.B) {
-        double dec = 0.0;
-        if (valid) dec = 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)l * bx.h * bx.a2, T.kmax));
-        bool two = (t0 + 1 < T.B) && (2 * tl + 1 < tb);
-        for (int e = 0; e < E; ++e) {
+    // deconvolution once per position
+    if (tid < np) {
+        const int pq = it.start + tid;
+        const int lq = pq <= bx.L ? pq : pq - bx.Nf;
+        s_dec[tid] = (lq >= -bx.L && lq <= bx.L)
+                         ? 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)lq * bx.h * bx.a2, T.kmax))
+                         : 0.0;
+    }
+    __syncthreads();
+    if (lane_on && t0 < T.B) {
+        const double dec = s_dec[pl];
+        const bool two = (t0 + 1 < T.B) && (2 * tl + 1 < tb);
+#pragma unroll
+        for (int e2 = 0; e2 < EG; ++e2) {
+            const int e = g * EG + e2;
+            if (e >= E) break;
             C2 *d0 = grid + bx.goff + ((long long)(e * T.B + t0)) * bx.Nf + p;
-            R v0r = (R)((double)a0r[e] * dec), v0i = (R)((double)a0i[e] * dec);
-            R v1r = (R)((double)a1r[e] * dec), v1i = (R)((double)a1i[e] * dec);
+            const R v0r = (R)((double)a0r[e2] * dec), v0i = (R)((double)a0i[e2] * dec);
+            const R v1r = (R)((double)a1r[e2] * dec), v1i = (R)((double)a1i[e2] * dec);
             if (it.pad) {
                 if (valid) {
                     grid_add(d0, v0r, v0i);

@@ -11,6 +11,7 @@
 #include "device_common.h"
 #include "fft_plan.h"
 #include "internal.h"
+#include "kernels_fused.h"
 
 namespace nufht_dev {
 
@@ -18,7 +19,7 @@ namespace nufht_dev {
 // per-stage timing
 // (one kernel per stage, except asy_fft: the rocFFT launches of all classes)
 enum Stage { ST_ZERO, ST_PERM, ST_DIR, ST_LOCP, ST_LOCE, ST_GZERO, ST_SPREAD, ST_SPREAD_MF,
-             ST_FFT, ST_INTERP, ST_INTERP_MF, ST_GRAD, ST_COUNT };
+             ST_FFT, ST_INTERP, ST_INTERP_MF, ST_GRAD, ST_FUSED, ST_COUNT };
 bool prof_enabled();
 void prof_push(hipEvent_t a, hipEvent_t b, int stage);
 
@@ -51,6 +52,12 @@ double knobs_default_dense_cell();
 constexpr int kSpreadSplit = 2048;
 // consecutive sources per matrix-core spread item (0: from the plan's size)
 constexpr int kDenseSplit = 0;
+// sources and targets of an asymptotic box with a small uniform grid up to
+// which it runs the fused kernel (kernels_fused.h); 0 by default: measured
+// slower than the staged kernels on every benchmark workload, the autotuner
+// tries it
+constexpr int kFusedMax = 0;
+int knobs_default_fused_max();   // kFusedMax, or NUFHT_AMD_FUSED_MAX
 
 // Launch-geometry choices of a program (NufhtAmdKnobs without precision).
 struct Knobs {
@@ -60,11 +67,13 @@ struct Knobs {
     int interp_mfma = 1;
     int own_fft = 1;
     int spread_sched = -1;   // matrix-core spread schedule: -1 automatic, 0 unified, 1 split
+    int fused_max = knobs_default_fused_max();   // 0: no fused boxes
     bool operator==(const Knobs &o) const
     {
         return dense_cell == o.dense_cell && dense_split == o.dense_split &&
                spread_split == o.spread_split && interp_mfma == o.interp_mfma &&
-               own_fft == o.own_fft && spread_sched == o.spread_sched;
+               own_fft == o.own_fft && spread_sched == o.spread_sched &&
+               fused_max == o.fused_max;
     }
     bool operator!=(const Knobs &o) const { return !(*this == o); }
 };
@@ -111,6 +120,15 @@ struct Program {
     int loc_ch = 1024, loc_tpb = 256, loc_nlt = 1;
     size_t loc_lds_e = 0;
     std::vector<AsyBoxD> abox;
+    std::vector<char> afused;         // box runs in the fused kernel
+    // fused small boxes: units (in decreasing cost), their grid passes, the
+    // work queue's counter, LDS grid elements per block, persistent blocks
+    std::vector<FusedUnit> funits;
+    std::vector<FftPass> fpass;
+    FusedUnit *d_funits = nullptr;
+    FftPass *d_fpass = nullptr;
+    int *d_fqueue = nullptr;
+    int fused_grid_elems = 0, fused_blocks = 0;
     AsyBoxD *d_abox = nullptr;
     std::vector<AsyItem> aspread, ainterp, azero, adense;
     std::vector<InterpRun> aimf;      // matrix-core interpolation runs
@@ -145,5 +163,6 @@ template <typename R> void asy_grid_zero(Program *P, hipStream_t st);
 template <typename R> void asy_fft(Program *P, hipStream_t st);
 template <typename R, int EM> void asy_spread(Program *P, hipStream_t st);
 template <typename R, int EM> void asy_interp(Program *P, hipStream_t st);
+template <typename R, int EM> void asy_fused(Program *P, hipStream_t st);
 
 }  // namespace nufht_dev
