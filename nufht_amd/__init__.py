@@ -503,8 +503,8 @@ def profile_read() -> Dict[str, Dict[str, float]]:
     ms = (ctypes.c_double * n)()
     cnt = (ctypes.c_longlong * n)()
     names = (ctypes.c_char_p * n)()
-    lib().nufht_amd_profile_read(ms, cnt, n, names)
-    return {names[q].decode(): {"ms": ms[q], "launches": cnt[q]} for q in range(n)}
+    m = min(n, int(lib().nufht_amd_profile_read(ms, cnt, n, names)))
+    return {names[q].decode(): {"ms": ms[q], "launches": cnt[q]} for q in range(m)}
 
 
 def version() -> str:

@@ -1330,10 +1330,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     RC_TRY(upload(&P->d_adrange, P->adrange));
     RC_TRY(upload(&P->d_funits, P->funits));
     if (!P->funits.empty()) {
-        HIP_TRY(hipMalloc((void **)&P->d_fqueue, sizeof(int)));
         int dev = 0, ncu = 0;
         HIP_TRY(hipGetDevice(&dev));
         HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HIP_TRY(hipMalloc((void **)&P->d_fqueue, sizeof(int)));
         P->fused_blocks = (int)std::min<size_t>(P->funits.size(), (size_t)std::max(1, 2 * ncu));
     }
     for (auto &it : P->ainterp) P->interp_tmax = std::max(P->interp_tmax, 256 / it.tb);
