@@ -2216,7 +2216,7 @@ extern "C" int nufht_multi_get_knobs(const NufhtMulti *mt, NufhtAmdKnobs *k)
 }
 
 // Candidate values per knob, tried in this order by coordinate descent.
-static const double kTuneDense[] = {2.0, 4.0, 8.0, 1e300};
+static const double kTuneDense[] = {2.0, 4.0, 8.0, 16.0, 32.0, 1e300};
 static const int kTuneDenseSplit[] = {512, 1024, 2048, 4096};
 static const int kTuneSpreadSplit[] = {1024, 2048, 4096};
 static const int kTuneFusedMax[] = {0, 128, 512};

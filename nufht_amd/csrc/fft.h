@@ -203,6 +203,92 @@ __device__ __forceinline__ void fft_lds(typename Cplx<R>::T *s, const FftPass &P
     }
 }
 
+// The first Stockham stage (Ns = 1) with its inputs taken straight from
+// global memory (ld(c, n): element n of sequence c) and its outputs written
+// to LDS, and the last stage (Ns = N / RAD) with its outputs handed to st(c,
+// n, v) instead of LDS: two LDS passes and one barrier fewer per block.
+// All of a thread's loads are issued before the first butterfly.
+template <typename R, int RAD, class Load>
+__device__ __forceinline__ void fft_first_stage(typename Cplx<R>::T *s, const FftPass &P, Load ld)
+{
+    typedef typename Cplx<R>::T C2;
+    constexpr int MAXB = (fft_max_elems<R>() / RAD + kFftThreads - 1) / kFftThreads;
+    const int nb = P.M * P.N / RAD, span = P.N / RAD;
+    C2 x[MAXB][RAD];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = threadIdx.x + q * kFftThreads;
+        if (b < nb) {
+            const int c = b & (P.M - 1), j = b >> P.lgM;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) x[q][r] = ld(c, j + r * span);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = threadIdx.x + q * kFftThreads;
+        if (b < nb) {
+            const int c = b & (P.M - 1), j = b >> P.lgM;
+            Dft<R, RAD>::run(x[q]);
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) s[fft_lds_idx(j * RAD + r, c, P.lds_stride, P.lds_shift)] = x[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename R, int RAD, class Store>
+__device__ __forceinline__ void fft_last_stage(const typename Cplx<R>::T *s, const FftPass &P,
+                                               const typename Cplx<R>::T *__restrict__ W, Store st)
+{
+    typedef typename Cplx<R>::T C2;
+    constexpr int MAXB = (fft_max_elems<R>() / RAD + kFftThreads - 1) / kFftThreads;
+    const int Ns = P.N / RAD, nb = P.M * Ns;
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = threadIdx.x + q * kFftThreads;
+        if (b < nb) {
+            const int c = b & (P.M - 1), j = b >> P.lgM;     // j < Ns: k = j
+            C2 x[RAD];
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) x[r] = s[fft_lds_idx(j + r * Ns, c, P.lds_stride, P.lds_shift)];
+#pragma unroll
+            for (int r = 1; r < RAD; ++r) x[r] = CMul<R>::mul(x[r], W[r * j]);
+            Dft<R, RAD>::run(x);
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) st(c, j + r * Ns, x[r]);
+        }
+    }
+}
+
+// The whole transform of a tile: first stage from global memory, the middle
+// stages in LDS, the last stage to global memory (at least two stages).
+template <typename R, class Load, class Store>
+__device__ __forceinline__ void fft_tile(typename Cplx<R>::T *s, const FftPass &P,
+                                         const typename Cplx<R>::T *__restrict__ W, Load ld, Store st)
+{
+    const int r0 = P.radix[0];
+    if (r0 == 8) fft_first_stage<R, 8>(s, P, ld);
+    else if (r0 == 4) fft_first_stage<R, 4>(s, P, ld);
+    else fft_first_stage<R, 2>(s, P, ld);
+    int lgNs = r0 == 8 ? 3 : r0 == 4 ? 2 : 1;
+    for (int q = 1; q + 1 < P.nstage; ++q) {
+        const int rad = P.radix[q];
+        if (rad == 8) fft_stage<R, 8>(s, P.N, P.M, P.lgM, lgNs, P.lds_stride, P.lds_shift, W);
+        else if (rad == 4) fft_stage<R, 4>(s, P.N, P.M, P.lgM, lgNs, P.lds_stride, P.lds_shift, W);
+        else if (rad == 3) fft_stage<R, 3>(s, P.N, P.M, P.lgM, lgNs, P.lds_stride, P.lds_shift, W);
+        else if (rad == 9) fft_stage<R, 9>(s, P.N, P.M, P.lgM, lgNs, P.lds_stride, P.lds_shift, W);
+        else fft_stage<R, 2>(s, P.N, P.M, P.lgM, lgNs, P.lds_stride, P.lds_shift, W);
+        lgNs += rad == 8 ? 3 : rad == 4 ? 2 : 1;
+    }
+    const int rl = P.radix[P.nstage - 1];
+    if (rl == 8) fft_last_stage<R, 8>(s, P, W, st);
+    else if (rl == 4) fft_last_stage<R, 4>(s, P, W, st);
+    else if (rl == 3) fft_last_stage<R, 3>(s, P, W, st);
+    else if (rl == 9) fft_last_stage<R, 9>(s, P, W, st);
+    else fft_last_stage<R, 2>(s, P, W, st);
+}
+
 // Blocks go to the 8 XCDs (each with its own L2) round robin in launch
 // order; a tile's runs of M elements fill half of a 128-byte line when M
 // elements are 64 bytes, the other half belonging to the next tile.  Within
@@ -227,36 +313,27 @@ __global__ __launch_bounds__(kFftThreads, 4) void k_fft_cols(typename Cplx<R>::T
     C2 *s = (C2 *)s_raw;
     const int bt = fft_block_tile();
     const int ch = bt / P.tiles, t = bt % P.tiles;
-    const int N = P.N, M = P.M;
+    const int M = P.M;
     const int L = chinfo[P.chinfo_off + ch].x;
     const int nf = (int)P.nf;
     C2 *base = g + (long long)ch * P.nf + (long long)t * M;
-    // all of a thread's loads in flight at once, then into LDS
-    constexpr int EPT = fft_max_elems<R>() / kFftThreads;
-    C2 v[EPT];
-#pragma unroll
-    for (int r = 0; r < EPT; ++r) {
-        int idx = threadIdx.x + r * kFftThreads;
-        int c = idx & (M - 1), n = idx >> P.lgM;
-        int p = n * P.N2 + t * M + c;
-        v[r].x = (R)0;
-        v[r].y = (R)0;
-        if (idx < M * N && (p <= L || p >= nf - L)) v[r] = base[(long long)n * P.N2 + c];
-    }
-#pragma unroll
-    for (int r = 0; r < EPT; ++r) {
-        int idx = threadIdx.x + r * kFftThreads;
-        if (idx < M * N) s[fft_lds_idx(idx >> P.lgM, idx & (M - 1), P.lds_stride, P.lds_shift)] = v[r];
-    }
-    __syncthreads();
-    fft_lds<R>(s, P, tw + P.w_off);
     const C2 *hi = tw + P.hi_off, *lo = tw + P.lo_off;
-    for (int idx = threadIdx.x; idx < M * N; idx += kFftThreads) {
-        int c = idx & (M - 1), k1 = idx >> P.lgM;
-        int a = k1 * (t * M + c);          // < (N1 - 1) (N2 - 1) < Nf
-        C2 w = CMul<R>::mul(hi[a >> 8], lo[a & 255]);
-        base[(long long)k1 * P.N2 + c] = CMul<R>::mul(s[fft_lds_idx(k1, c, P.lds_stride, P.lds_shift)], w);
-    }
+    // element n of column c: grid position p = n N2 + t M + c, zero (not
+    // stored) outside |l| <= L; out: times w^{k1 (t M + c)}, in place
+    auto ld = [&](int c, int n) {
+        const int p = n * P.N2 + t * M + c;
+        C2 v;
+        v.x = (R)0;
+        v.y = (R)0;
+        if (p <= L || p >= nf - L) v = base[(long long)n * P.N2 + c];
+        return v;
+    };
+    auto st = [&](int c, int k1, C2 v) {
+        const int a = k1 * (t * M + c);          // < (N1 - 1) (N2 - 1) < Nf
+        const C2 w = CMul<R>::mul(hi[a >> 8], lo[a & 255]);
+        base[(long long)k1 * P.N2 + c] = CMul<R>::mul(v, w);
+    };
+    fft_tile<R>(s, P, tw + P.w_off, ld, st);
 }
 
 // Pass 2: row FFTs of length N2, written to natural order in out.
@@ -272,38 +349,21 @@ __global__ __launch_bounds__(kFftThreads, 4) void k_fft_rows(const typename Cplx
     C2 *s = (C2 *)s_raw;
     const int bt = fft_block_tile();
     const int ch = bt / P.tiles, t = bt % P.tiles;
-    const int N = P.N, M = P.M;
+    const int M = P.M;
     const C2 *src = g + (long long)ch * P.nf + (long long)t * M * P.N2;
-    constexpr int EPT = fft_max_elems<R>() / kFftThreads, HALF = EPT / 2;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        C2 v[HALF];
-#pragma unroll
-        for (int r = 0; r < HALF; ++r) {
-            int idx = threadIdx.x + (h * HALF + r) * kFftThreads;
-            int n = idx & (N - 1), c = idx >> P.lgN;     // N = N2, a power of 2
-            v[r].x = (R)0;
-            v[r].y = (R)0;
-            if (idx < M * N) v[r] = src[(long long)c * P.N2 + n];
-        }
-#pragma unroll
-        for (int r = 0; r < HALF; ++r) {
-            int idx = threadIdx.x + (h * HALF + r) * kFftThreads;
-            if (idx < M * N) s[fft_lds_idx(idx & (N - 1), idx >> P.lgN, P.lds_stride, P.lds_shift)] = v[r];
-        }
-    }
-    __syncthreads();
-    fft_lds<R>(s, P, tw + P.w_off);
     C2 *dst = out + (long long)ch * P.nf + (long long)t * M;
     const int4 ci = chinfo[P.chinfo_off + ch];
     const int nf = (int)P.nf;
-    for (int idx = threadIdx.x; idx < M * N; idx += kFftThreads) {
-        int c = idx & (M - 1), k2 = idx >> P.lgM;
-        int q = t * M + c + k2 * P.N1;
+    auto ld = [&](int c, int n) { return src[(long long)c * P.N2 + n]; };
+    // output k2 of row t M + c is frequency q = t M + c + k2 N1; only the
+    // interpolation window [mlo, mlo + span] (mod Nf) is written
+    auto st = [&](int c, int k2, C2 v) {
+        const int q = t * M + c + k2 * P.N1;
         int d = q - ci.y;                  // mlo in (-Nf, Nf)
         d = d < 0 ? d + nf : (d >= nf ? d - nf : d);
-        if (d <= ci.z) dst[(long long)k2 * P.N1 + c] = s[fft_lds_idx(k2, c, P.lds_stride, P.lds_shift)];
-    }
+        if (d <= ci.z) dst[(long long)k2 * P.N1 + c] = v;
+    };
+    fft_tile<R>(s, P, tw + P.w_off, ld, st);
 }
 
 // Lengths up to one tile (Nf <= 4096 in FP64, 8192 in FP32): one pass, M
