@@ -228,6 +228,19 @@ def run_headline(args, dev):
     roof["achieved_path"] = path["achieved"]
     roof["path"] = path
     fft = fft_stage(prof, works)
+    # the same kernel and FFT stage alone: a few steps with the near-field
+    # stages on the calling stream (in the timed steps they run on a second
+    # stream and share the GPU with the asymptotic kernels)
+    prev = na.stream_fork(0)
+    prof1 = _profiled(step, 3)
+    na.stream_fork(prev)
+    r1 = roofline(prof1, works, mixed, None)
+    f1 = fft_stage(prof1, works)
+    roof["alone"] = {"avg_launch_ms": r1["avg_launch_ms"], "achieved": r1["achieved"],
+                     "frac": r1["frac"], "kernel": r1["kernel"],
+                     "fft_stage_ms": f1["avg_stage_ms"] if f1 else None,
+                     "fft_frac": f1["frac"] if f1 else None,
+                     "stages_ms_per_step": _stages(prof1, 3)}
 
     plain = None
     if args.plain_steps > 0:

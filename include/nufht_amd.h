@@ -103,6 +103,13 @@ void nufht_multi_free(NufhtMulti *mt);
  * nufht_amd_profile_read returns accumulated milliseconds and launch counts
  * per stage (after synchronising), and the stage names. */
 int  nufht_amd_profile(int enable);
+/* Where calls fork the near-field stages (local expansions, direct sums)
+ * onto the program's second stream: 1 before the spreads (default), 2 before
+ * the FFTs, 0 never (one stream: every stage then runs alone, so per-stage
+ * timings measure each kernel without the other stream's contention).  Takes
+ * effect for calls made afterwards; at < 0 only queries.  Returns the
+ * previous setting.  (NUFHT_AMD_FORK sets the initial value.) */
+int  nufht_amd_stream_fork(int at);
 int  nufht_amd_profile_read(double *ms, long long *launches, int max_stages,
                             const char **names);
 void nufht_amd_profile_reset(void);

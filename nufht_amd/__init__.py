@@ -113,6 +113,7 @@ EXPORTS = [
     "nufht_amd_last_error", "nufht_amd_get_knobs", "nufht_amd_set_knobs",
     "nufht_multi_autotune", "nufht_multi_get_knobs", "nufht_amd_plan_cache",
     "nufht_amd_write_table_cache", "nufht_batch_device_noalloc", "nufht_multi_stats",
+    "nufht_amd_stream_fork",
 ]
 
 
@@ -159,6 +160,7 @@ def lib():
                                       ctypes.POINTER(vp), ctypes.POINTER(i), vp, i]
     L.nufht_multi_free.argtypes = [vp]
     L.nufht_amd_profile.argtypes = [i]
+    L.nufht_amd_stream_fork.argtypes = [i]
     L.nufht_amd_profile_read.argtypes = [ctypes.POINTER(d), ctypes.POINTER(ctypes.c_longlong),
                                          i, ctypes.POINTER(ctypes.c_char_p)]
     L.nufht_amd_plan_stats.argtypes = [PP, i, ctypes.POINTER(PlanStats)]
@@ -488,6 +490,13 @@ class MultiBatch:
 STAGES = ["zero", "permute", "direct", "loc_project", "loc_eval", "asy_grid_zero",
           "asy_spread", "asy_spread_mfma", "asy_fft", "asy_interp", "asy_interp_mfma",
           "grad_combine", "asy_fused"]
+
+
+def stream_fork(at: int = -1) -> int:
+    """Where calls fork the near-field stages onto a second stream (1 before
+    the spreads, 2 before the FFTs, 0 never); at < 0 only queries.  Returns
+    the previous setting."""
+    return int(lib().nufht_amd_stream_fork(int(at)))
 
 
 def profile(enable: bool = True):

@@ -1515,6 +1515,13 @@ static int g_fork_at = [] {
     return e ? atoi(e) : 1;
 }();
 
+extern "C" int nufht_amd_stream_fork(int at)
+{
+    const int prev = g_fork_at;
+    if (at >= 0 && at <= 2) g_fork_at = at;
+    return prev;
+}
+
 // Near-field stages: local projection and evaluation, then the direct sums.
 static int run_near_field(Program *P, hipStream_t st, double *coef)
 {
