@@ -1312,6 +1312,22 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     clk.mark("work lists (host)");
     // uploads
     RC_TRY(upload(&P->d_zero, P->zero));
+    {
+        // direct items grouped (stably) by column class: <= 4, <= 8, <= 16 columns
+        auto cls = [&](const DirItem &d) {
+            const int nb = std::min(kDirBM, P->tfs[d.tf].B - d.col0);
+            return nb <= 4 ? 0 : nb <= 8 ? 1 : 2;
+        };
+        std::vector<DirItem> d2;
+        d2.reserve(P->dir.size());
+        for (int c = 0; c < 3; ++c) {
+            P->dir_off[c] = (int)d2.size();
+            for (auto &d : P->dir)
+                if (cls(d) == c) d2.push_back(d);
+        }
+        P->dir_off[3] = (int)d2.size();
+        P->dir.swap(d2);
+    }
     RC_TRY(upload(&P->d_dir, P->dir));
     RC_TRY(upload(&P->d_dir_rng, P->dir_rng));
     RC_TRY(upload(&P->d_lbox, P->lbox));
@@ -1556,8 +1572,17 @@ static int run_near_field(Program *P, hipStream_t st, double *coef)
     }
     if (!P->dir.empty()) {
         StageTimer tm(ST_DIR, st);
-        hipLaunchKernelGGL(k_dir, dim3((unsigned)P->dir.size()), dim3(256), 0, st, P->d_tfs,
-                           P->d_dir, P->d_dir_rng, P->d_jtab);
+        // items grouped by column class (4, 8, 16), one launch per class
+        const int *o = P->dir_off;
+        if (o[1] > o[0])
+            hipLaunchKernelGGL(k_dir<4>, dim3((unsigned)(o[1] - o[0])), dim3(256), 0, st, P->d_tfs,
+                               P->d_dir + o[0], P->d_dir_rng, P->d_jtab);
+        if (o[2] > o[1])
+            hipLaunchKernelGGL(k_dir<8>, dim3((unsigned)(o[2] - o[1])), dim3(256), 0, st, P->d_tfs,
+                               P->d_dir + o[1], P->d_dir_rng, P->d_jtab);
+        if (o[3] > o[2])
+            hipLaunchKernelGGL(k_dir<16>, dim3((unsigned)(o[3] - o[2])), dim3(256), 0, st, P->d_tfs,
+                               P->d_dir + o[2], P->d_dir_rng, P->d_jtab);
     }
     return 0;
 }

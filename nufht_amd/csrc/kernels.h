@@ -66,26 +66,37 @@ __global__ void k_jtab_build(int order, int half, int nint, double *__restrict__
     tab[(long long)iv * N + k] = acc * (k == 0 ? 1.0 : 2.0) / N;
 }
 
+// NB: the item's columns, rounded up to 4, 8 or 16 (one kernel per class).
+// The tpr lanes of a row each take a contiguous, odd-length share of the
+// staged sources (odd: the lanes' reads fall in different LDS banks); with
+// sources sorted by r, x = k r rises slowly along a share, so the lane keeps
+// the Chebyshev coefficients of its current unit interval in registers and
+// reloads them only when x enters the next one.
+template <int NB>
 __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
                                              const DirItem *__restrict__ items,
                                              const int2 *__restrict__ rng,
                                              const double *__restrict__ jtab)
 {
-    constexpr int LDC = kDirBM + 1;                 // padded row: no bank conflicts
+    constexpr int LDC = NB + 1;                     // padded row: no bank conflicts
     __shared__ double s_r[kDirTJ];
     __shared__ double s_c[kDirTJ * LDC];
     DirItem it = items[blockIdx.x];
     const TfDesc &T = tfs[it.tf];
-    int nb = min(kDirBM, T.B - it.col0);
+    int nb = min(NB, T.B - it.col0);
     int tpr = it.tpr;
     int row = threadIdx.x / tpr, lane = threadIdx.x % tpr;
     bool active = row < it.ni;
-    double acc[kDirBM];
+    double acc[NB];
 #pragma unroll
-    for (int t = 0; t < kDirBM; ++t) acc[t] = 0.0;
+    for (int t = 0; t < NB; ++t) acc[t] = 0.0;
     const double *tab = jtab + T.jtab_off;
     double xmax = (double)T.jtab_n;
     const double w = active ? T.w[it.i0 + row] : 0.0;
+    int civ = -1;                   // unit interval of the coefficients held in cc
+    double cc[kJDeg + 1];
+#pragma unroll
+    for (int n = 0; n <= kJDeg; ++n) cc[n] = 0.0;
     for (int q = 0; q < it.rcnt; ++q) {
         const int2 rq = rng[it.roff + q];
         for (int c = 0; c < rq.y; c += kDirTJ) {
@@ -95,34 +106,51 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
                 // one source per thread (nj <= kDirTJ = blockDim), its columns loaded together
                 const int j = threadIdx.x;
                 const bool ok = j < nj;
-                double cv[kDirBM];
+                double cv[NB];
 #pragma unroll
-                for (int t = 0; t < kDirBM; ++t)
+                for (int t = 0; t < NB; ++t)
                     cv[t] = (ok && t < nb) ? T.cs[(long long)(it.col0 + t) * T.ld_cs + j0 + j] : 0.0;
                 s_r[j] = ok ? T.r[j0 + j] : 0.0;
 #pragma unroll
-                for (int t = 0; t < kDirBM; ++t) s_c[j * LDC + t] = cv[t];
+                for (int t = 0; t < NB; ++t) s_c[j * LDC + t] = cv[t];
             }
             __syncthreads();
             if (active) {
-                for (int j = lane; j < nj; j += tpr) {
-                    double x = w * s_r[j];
+                const int per = ((nj + tpr - 1) / tpr) | 1;
+                const int ja = lane * per, jb = min(nj, ja + per);
+                for (int j = ja; j < jb; ++j) {
+                    const double x = w * s_r[j];
                     double J;
                     if (x >= 0.0 && x < xmax) {
-                        J = jtab_eval(tab, x);
+                        const int iv = (int)x;
+                        if (iv != civ) {
+                            civ = iv;
+                            const double *cf = tab + (long long)iv * (kJDeg + 1);
+#pragma unroll
+                            for (int n = 0; n <= kJDeg; ++n) cc[n] = cf[n];
+                        }
+                        const double tt = 2.0 * (x - (double)iv) - 1.0;
+                        double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+                        for (int n = kJDeg; n >= 1; --n) {
+                            const double b0 = fma(2.0 * tt, b1, cc[n] - b2);
+                            b2 = b1;
+                            b1 = b0;
+                        }
+                        J = fma(tt, b1, cc[0] - b2);
                         if (T.half) J *= sqrt(x);
                     } else {
                         J = besselj_wr(T.order, T.half, w, s_r[j]);
                     }
 #pragma unroll
-                    for (int t = 0; t < kDirBM; ++t) acc[t] = fma(J, s_c[j * LDC + t], acc[t]);
+                    for (int t = 0; t < NB; ++t) acc[t] = fma(J, s_c[j * LDC + t], acc[t]);
                 }
             }
         }
     }
     for (int off = tpr >> 1; off > 0; off >>= 1) {
 #pragma unroll
-        for (int t = 0; t < kDirBM; ++t) acc[t] += __shfl_xor(acc[t], off, 64);
+        for (int t = 0; t < NB; ++t) acc[t] += __shfl_xor(acc[t], off, 64);
     }
     if (active && lane == 0) {
         for (int t = 0; t < nb; ++t)

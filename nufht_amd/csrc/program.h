@@ -103,6 +103,7 @@ struct Program {
     std::vector<DirItem> dir;
     DirItem *d_dir = nullptr;
     std::vector<int2> dir_rng;        // source ranges {j0, count} of the direct items
+    int dir_off[4] = {0, 0, 0, 0};    // direct items with <= 4, <= 8, <= 16 columns: [off[c], off[c+1])
     int2 *d_dir_rng = nullptr;
     std::vector<LocBoxD> lbox;
     LocBoxD *d_lbox = nullptr;
