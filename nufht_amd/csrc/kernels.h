@@ -78,9 +78,11 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
                                              const int2 *__restrict__ rng,
                                              const double *__restrict__ jtab)
 {
-    constexpr int LDC = NB + 1;                     // padded row: no bank conflicts
+    // rows of NB + 2 doubles: 16-byte aligned for paired reads, and the odd
+    // shares of the row's lanes still fall in different banks
+    constexpr int LDC = NB + 2;
     __shared__ double s_r[kDirTJ];
-    __shared__ double s_c[kDirTJ * LDC];
+    __shared__ __align__(16) double s_c[kDirTJ * LDC];
     DirItem it = items[blockIdx.x];
     const TfDesc &T = tfs[it.tf];
     int nb = min(NB, T.B - it.col0);
@@ -112,7 +114,7 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
                     cv[t] = (ok && t < nb) ? T.cs[(long long)(it.col0 + t) * T.ld_cs + j0 + j] : 0.0;
                 s_r[j] = ok ? T.r[j0 + j] : 0.0;
 #pragma unroll
-                for (int t = 0; t < NB; ++t) s_c[j * LDC + t] = cv[t];
+                for (int t = 0; t < NB; t += 2) *(double2 *)&s_c[j * LDC + t] = make_double2(cv[t], cv[t + 1]);
             }
             __syncthreads();
             if (active) {
@@ -142,8 +144,13 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
                     } else {
                         J = besselj_wr(T.order, T.half, w, s_r[j]);
                     }
+                    const double2 *cr = (const double2 *)&s_c[j * LDC];
 #pragma unroll
-                    for (int t = 0; t < NB; ++t) acc[t] = fma(J, s_c[j * LDC + t], acc[t]);
+                    for (int t = 0; t < NB / 2; ++t) {
+                        const double2 v = cr[t];
+                        acc[2 * t] = fma(J, v.x, acc[2 * t]);
+                        acc[2 * t + 1] = fma(J, v.y, acc[2 * t + 1]);
+                    }
                 }
             }
         }
