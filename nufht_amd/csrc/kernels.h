@@ -120,30 +120,76 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
             if (active) {
                 const int per = ((nj + tpr - 1) / tpr) | 1;
                 const int ja = lane * per, jb = min(nj, ja + per);
-                for (int j = ja; j < jb; ++j) {
-                    const double x = w * s_r[j];
-                    double J;
-                    if (x >= 0.0 && x < xmax) {
-                        const int iv = (int)x;
-                        if (iv != civ) {
-                            civ = iv;
-                            const double *cf = tab + (long long)iv * (kJDeg + 1);
+                auto load_iv = [&](int iv) {
+                    if (iv != civ) {
+                        civ = iv;
+                        const double *cf = tab + (long long)iv * (kJDeg + 1);
 #pragma unroll
-                            for (int n = 0; n <= kJDeg; ++n) cc[n] = cf[n];
-                        }
-                        const double tt = 2.0 * (x - (double)iv) - 1.0;
-                        double b1 = 0.0, b2 = 0.0;
+                        for (int n = 0; n <= kJDeg; ++n) cc[n] = cf[n];
+                    }
+                };
+                // J at x = w r_j: the table (its interval's coefficients in
+                // registers), or the exact routine beyond it
+                auto one = [&](int j) {
+                    const double x = w * s_r[j];
+                    if (!(x >= 0.0 && x < xmax)) return besselj_wr(T.order, T.half, w, s_r[j]);
+                    const int iv = (int)x;
+                    load_iv(iv);
+                    const double tt = 2.0 * (x - (double)iv) - 1.0;
+                    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+                    for (int n = kJDeg; n >= 1; --n) {
+                        const double b0 = fma(2.0 * tt, b1, cc[n] - b2);
+                        b2 = b1;
+                        b1 = b0;
+                    }
+                    const double J = fma(tt, b1, cc[0] - b2);
+                    return T.half ? J * sqrt(x) : J;
+                };
+                int j = ja;
+                // 16 columns: two sources at a time; in one interval (the
+                // common case) their Clenshaw recurrences run interleaved
+                // (for fewer columns the second chain's registers cost more
+                // occupancy than the interleaving gains)
+                for (; NB == 16 && j + 1 < jb; j += 2) {
+                    const double x0 = w * s_r[j], x1 = w * s_r[j + 1];
+                    double J0, J1;
+                    const int iv0 = (int)x0;
+                    if (x0 >= 0.0 && x1 < xmax && iv0 == (int)x1) {
+                        load_iv(iv0);
+                        const double t0 = 2.0 * (x0 - (double)iv0) - 1.0;
+                        const double t1 = 2.0 * (x1 - (double)iv0) - 1.0;
+                        double a1 = 0.0, a2 = 0.0, c1 = 0.0, c2 = 0.0;
 #pragma unroll
                         for (int n = kJDeg; n >= 1; --n) {
-                            const double b0 = fma(2.0 * tt, b1, cc[n] - b2);
-                            b2 = b1;
-                            b1 = b0;
+                            const double a0 = fma(2.0 * t0, a1, cc[n] - a2);
+                            const double c0 = fma(2.0 * t1, c1, cc[n] - c2);
+                            a2 = a1;
+                            a1 = a0;
+                            c2 = c1;
+                            c1 = c0;
                         }
-                        J = fma(tt, b1, cc[0] - b2);
-                        if (T.half) J *= sqrt(x);
+                        J0 = fma(t0, a1, cc[0] - a2);
+                        J1 = fma(t1, c1, cc[0] - c2);
+                        if (T.half) {
+                            J0 *= sqrt(x0);
+                            J1 *= sqrt(x1);
+                        }
                     } else {
-                        J = besselj_wr(T.order, T.half, w, s_r[j]);
+                        J0 = one(j);
+                        J1 = one(j + 1);
                     }
+                    const double2 *cr0 = (const double2 *)&s_c[j * LDC];
+                    const double2 *cr1 = (const double2 *)&s_c[(j + 1) * LDC];
+#pragma unroll
+                    for (int t = 0; t < NB / 2; ++t) {
+                        const double2 v = cr0[t], u = cr1[t];
+                        acc[2 * t] = fma(J1, u.x, fma(J0, v.x, acc[2 * t]));
+                        acc[2 * t + 1] = fma(J1, u.y, fma(J0, v.y, acc[2 * t + 1]));
+                    }
+                }
+                for (; j < jb; ++j) {
+                    const double J = one(j);
                     const double2 *cr = (const double2 *)&s_c[j * LDC];
 #pragma unroll
                     for (int t = 0; t < NB / 2; ++t) {
