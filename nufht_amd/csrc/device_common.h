@@ -134,6 +134,30 @@ __device__ __forceinline__ void sincos_cw(double v, double *s, double *c)
     *c = ((q + 1) & 2) ? -cc : cc;
 }
 
+// sph_jn (bessel_dev.h) with the Cody-Waite reduction above (its full-range
+// fallback out of line) instead of the library's sincos: the local
+// expansions' arguments stay below the split point z_split, far inside the
+// range where the reduction is exact to an ulp.
+__device__ inline double sph_jn_cw(int n, double hi, double lo)
+{
+    const double x = hi;
+    if (x == 0.0) return n == 0 ? 1.0 : 0.0;
+    if (x <= fmax(2.0, (double)n)) return series_j(n, 1, x) * sqrt(kPi / (2.0 * x));
+    double sh, ch;
+    sincos_cw<SC_CALL>(hi, &sh, &ch);
+    const double s = fma(lo, ch, sh), c = fma(-lo, sh, ch);
+    const double xi = 1.0 / x;
+    double j0 = s * xi;
+    if (n == 0) return j0;
+    double j1 = (s * xi - c) * xi;
+    for (int k = 1; k < n; ++k) {
+        const double j2 = (double)(2 * k + 1) * xi * j1 - j0;
+        j0 = j1;
+        j1 = j2;
+    }
+    return j1;
+}
+
 // Kernel values at the kw + 1 grid points l = lf + k of a point x, with
 // lf = ceil((x - a1) / h): psi_k(f) = ES((2 / kw) (k - kw / 2 + f)) for the
 // cell offset f = (lf h - x) / h + kw / 2 in [0, 1), each tap a Chebyshev

@@ -982,6 +982,8 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                         else v = (4.0 / N) * std::cos((2 * l + p) * th) / 2.0;
                         cheb_all.push_back(v);
                     }
+                // the nodes x_q = cos(pi (q + 1/2) / N) after the matrix
+                for (int q = 0; q <= K; ++q) cheb_all.push_back(std::cos(M_PI * (q + 0.5) / N));
             }
         }
         for (auto &b : locb) {

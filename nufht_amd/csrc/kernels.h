@@ -347,6 +347,31 @@ __global__ __launch_bounds__(256) void k_loc_proj(const TfDesc *__restrict__ tfs
 // [slot][thread] (slots = loc_slots()), so lanes touch consecutive words;
 // a box's coefficients are the same for the whole block and are read as
 // uniform loads.
+// b_l = sum_q M[l][q] f_q for l <= K, f_q at slots K + 1 + q, b_l to slots
+// l: four rows at a time (independent accumulators, one read of f_q each)
+__device__ __forceinline__ void cheb_apply(double *S, int nthr, int K, const double *__restrict__ M)
+{
+    const int n = K + 1;
+    for (int l = 0; l <= K; l += 4) {
+        const double *m0 = M + (long long)l * n;
+        const double *m1 = l + 1 <= K ? m0 + n : m0;
+        const double *m2 = l + 2 <= K ? m0 + 2 * n : m0;
+        const double *m3 = l + 3 <= K ? m0 + 3 * n : m0;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (int q = 0; q <= K; ++q) {
+            const double f = S[(n + q) * nthr];
+            a0 = fma(m0[q], f, a0);
+            a1 = fma(m1[q], f, a1);
+            a2 = fma(m2[q], f, a2);
+            a3 = fma(m3[q], f, a3);
+        }
+        S[l * nthr] = a0;
+        if (l + 1 <= K) S[(l + 1) * nthr] = a1;
+        if (l + 2 <= K) S[(l + 2) * nthr] = a2;
+        if (l + 3 <= K) S[(l + 3) * nthr] = a3;
+    }
+}
+
 __host__ __device__ inline int loc_slots(int order, int half, int K)
 {
     return half ? 2 * (K + 1) : (order >> 1) + K + (order & 1) + 1;
@@ -367,12 +392,16 @@ __global__ __launch_bounds__(256) void k_loc_eval1(const TfDesc *__restrict__ tf
     double *S = sm + tid;   // slot k at S[k * nthr]
     const int i = bx.i0 + it.start + tid;
     const double w = T.w[i];
-    double scale = 1.0;
-    int boff;               // slot of b_0 (b_l at slot boff + l)
+    const double sc0 = T.sign;
+    const int B = T.B;
     if (!T.half) {
+        // J_0..J_maxo by one Miller sweep, left unnormalised in S (the
+        // normalisation 1/(sum + j) is applied to both factors of each b_l
+        // in the contraction: no separate LDS pass)
         const int l0 = T.order >> 1, odd = T.order & 1;
         const int maxo = l0 + K + odd;
         const double z = w * bx.r_end * 0.5;
+        double inv = 1.0;
         if (z == 0.0) {
             for (int k = 0; k <= maxo; ++k) S[k * nthr] = (k == 0) ? 1.0 : 0.0;
         } else {
@@ -392,40 +421,52 @@ __global__ __launch_bounds__(256) void k_loc_eval1(const TfDesc *__restrict__ tf
                     for (int q = km1; q <= maxo; ++q) S[q * nthr] *= 1e-200;
                 }
             }
-            double inv = 1.0 / (sum + j);
-            for (int k = 0; k <= maxo; ++k) S[k * nthr] *= inv;
+            inv = 1.0 / (sum + j);
         }
-        // b_l replaces J_{l0+odd+l}; taken from the top down, every J still
-        // needed sits below the slots already replaced
-        for (int l = K; l >= 0; --l) {
-            int lo = l0 - l;
-            lo = lo < 0 ? -lo : lo;
-            double v = S[(l0 + odd + l) * nthr] * S[lo * nthr];
-            if (l > l0 && ((l - l0) & 1)) v = -v;
-            S[(l0 + odd + l) * nthr] = v;
+        // b_l = J_{l0+odd+l} J_{|l0-l|} (sign (-1) for l > l0 with l - l0 odd)
+        for (int t0 = 0; t0 < B; t0 += 4) {
+            const int nt = min(4, B - t0);
+            LR a0 = (LR)0, a1 = (LR)0, a2 = (LR)0, a3 = (LR)0;
+            const double *cf = coef + bx.coef_off + t0;
+            for (int l = 0; l <= K; ++l) {
+                int lo = l0 - l;
+                lo = lo < 0 ? -lo : lo;
+                double v = (S[(l0 + odd + l) * nthr] * inv) * (S[lo * nthr] * inv);
+                if (l > l0 && ((l - l0) & 1)) v = -v;
+                const LR bl = (LR)v;
+                const double *cr = cf + (long long)l * B;
+                a0 = fma(bl, (LR)cr[0], a0);
+                if (nt > 1) a1 = fma(bl, (LR)cr[1], a1);
+                if (nt > 2) a2 = fma(bl, (LR)cr[2], a2);
+                if (nt > 3) a3 = fma(bl, (LR)cr[3], a3);
+            }
+            double *g = T.gs + (long long)t0 * T.ld_gs + i;
+            atomicAdd(g, sc0 * (double)a0);
+            if (nt > 1) atomicAdd(g + T.ld_gs, sc0 * (double)a1);
+            if (nt > 2) atomicAdd(g + 2 * (long long)T.ld_gs, sc0 * (double)a2);
+            if (nt > 3) atomicAdd(g + 3 * (long long)T.ld_gs, sc0 * (double)a3);
         }
-        boff = l0 + odd;
-    } else {
+        return;
+    }
+    double scale = 1.0;
+    int boff;               // slot of b_0 (b_l at slot boff + l)
+    {
         // f_q = j_n(z x_q), x_q = cos(pi (q + 1/2) / (2K + 2)), q = 0..K, at
         // slots K+1+q; b_l at slots l
         const double z = w * bx.r_end;
-        for (int q = 0; q <= K; ++q) {
-            double xq = cos(kPi * ((double)q + 0.5) / (double)(2 * K + 2));
-            double hi = z * xq, lo = fma(z, xq, -hi);
-            S[(K + 1 + q) * nthr] = sph_jn(T.order, hi, lo);
-        }
         const double *M = cheb + T.cheb_off;
-        for (int l = 0; l <= K; ++l) {
-            double a = 0.0;
-            for (int q = 0; q <= K; ++q) a = fma(M[l * (K + 1) + q], S[(K + 1 + q) * nthr], a);
-            S[l * nthr] = a;
+        const double *xn = M + (K + 1) * (K + 1);     // the nodes x_q, after the matrix
+        for (int q = 0; q <= K; ++q) {
+            const double xq = xn[q];
+            double hi = z * xq, lo = fma(z, xq, -hi);
+            S[(K + 1 + q) * nthr] = sph_jn_cw(T.order, hi, lo);
         }
+        cheb_apply(S, nthr, K, M);
         boff = 0;
         scale = sqrt(2.0 * w / kPi);
     }
-    const double sc = T.sign * scale;
+    const double sc = sc0 * scale;
     const double *cf = coef + bx.coef_off;
-    const int B = T.B;
     for (int t0 = 0; t0 < B; t0 += 4) {
         const int nt = min(4, B - t0);
         LR a0 = (LR)0, a1 = (LR)0, a2 = (LR)0, a3 = (LR)0;
@@ -507,17 +548,14 @@ __global__ __launch_bounds__(256) void k_loc_eval(const TfDesc *__restrict__ tfs
             // f_q = j_n(z x_q), x_q = cos(pi (q + 1/2) / (2K + 2)), q = 0..K, at
             // slots K+1+q; b_l at slots l
             const double z = w * bx.r_end;
-            for (int qq = 0; qq <= K; ++qq) {
-                double xq = cos(kPi * ((double)qq + 0.5) / (double)(2 * K + 2));
-                double hi = z * xq, lo = fma(z, xq, -hi);
-                S[(K + 1 + qq) * nthr] = sph_jn(T.order, hi, lo);
-            }
             const double *M = cheb + T.cheb_off;
-            for (int l = 0; l <= K; ++l) {
-                double a = 0.0;
-                for (int qq = 0; qq <= K; ++qq) a = fma(M[l * (K + 1) + qq], S[(K + 1 + qq) * nthr], a);
-                S[l * nthr] = a;
+            const double *xn = M + (K + 1) * (K + 1);   // the nodes x_q, after the matrix
+            for (int qq = 0; qq <= K; ++qq) {
+                const double xq = xn[qq];
+                double hi = z * xq, lo = fma(z, xq, -hi);
+                S[(K + 1 + qq) * nthr] = sph_jn_cw(T.order, hi, lo);
             }
+            cheb_apply(S, nthr, K, M);
             boff = 0;
         }
         const double *cf = coef + bx.coef_off + it.col0;
