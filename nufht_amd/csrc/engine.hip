@@ -1094,6 +1094,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
             }
         }
         c.p1.nch = c.p2.nch = (int)c.batch;
+        if (g_trace)
+            fprintf(stderr, "[nufht_amd] fft class Nf %d channels %lld %s N1 %d M1 %d N2 %d M2 %d\n",
+                    c.nf, c.batch, !c.own ? "rocfft" : c.small ? "one-pass" : "two-pass",
+                    c.small ? c.nf : c.p1.N1, c.p1.M, c.small ? 0 : c.p2.N2, c.small ? 0 : c.p2.M);
         P->fft.push_back(c);
     }
     P->grid_elems = off;
