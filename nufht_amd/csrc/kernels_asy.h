@@ -625,11 +625,16 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
     __shared__ int s_m0[kIrTargets];
     __shared__ double s_P[kIrTargets][EMAX];
     __shared__ double s_out[4][16][17];           // per wave: result tile, transposed
+    __shared__ __align__(16) double s_tap[kTapMax][16];   // kernel tap series, rows padded
     const InterpRun it = items[xcd_item<4>()];
     const AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int E = bx.E, kw = T.kw;
+    for (int q = tid; q < kw * kEsN; q += 256) {
+        const int k = q / kEsN;
+        s_tap[k][q - k * kEsN] = ft[T.ft_off + kNFT + q];
+    }
     // per-target factors, one thread per target (rows past the run's end
     // repeat its last target with zero weight)
     for (int ti = tid; ti < kIrTargets; ti += 256) {
@@ -678,11 +683,19 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
             // its m0 + kw - 1): the others are all zero and are skipped
             const int nsteps = min(4, (s_m0[b0 + cnt - 1] + kw - mw + 3) >> 2);
             nst[u] = nsteps;
+            // phi(s - m Delta) at tap k = m - m0 of the target (zero outside
+            // 0 <= k < kw) from the tap's Chebyshev series in the cell offset
+            // f = kw / 2 - (s - m0 Delta) / Delta
+            const int mi = s_m0[b0 + ia];
+            const double f = 0.5 * kw - (s_s[b0 + ia] - (double)mi * bx.delta) / bx.delta;
+            R tcb[kEsN];
+            cheb_basis((R)(2.0 * f - 1.0), tcb);
 #pragma unroll
             for (int st = 0; st < 4; ++st) {
-                const int m = mw + 4 * st + ka;
-                const double x = (s_s[b0 + ia] - (double)m * bx.delta) / bx.a2;
-                phiA[u][st] = (st < nsteps && ia < cnt) ? (R)es_kernel(x, T.beta) : (R)0;
+                const int k = mw + 4 * st + ka - mi;
+                const bool on = st < nsteps && ia < cnt && k >= 0 && k < kw;
+                const R v = es_tap(&s_tap[on ? k : 0][0], tcb);
+                phiA[u][st] = on ? v : (R)0;
             }
         }
     }
@@ -692,14 +705,13 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
     // group's products, and written to LDS between two barriers
     constexpr int PF = 16 * kIrPos / 256;
     C2 pf[PF];
-    int pc[PF], pp_[PF];     // channel and position of this thread's r-th value
+    int pcp[PF];     // channel (high half) and position (low half) of this thread's r-th value
     {
         const int dc = 256 / it.npos, dp = 256 - dc * it.npos;
         int c = tid / it.npos, p = tid - c * it.npos;
 #pragma unroll
         for (int r = 0; r < PF; ++r) {
-            pc[r] = c;
-            pp_[r] = p;
+            pcp[r] = (c << 16) | p;
             c += dc;
             p += dp;
             if (p >= it.npos) { p -= it.npos; ++c; }
@@ -710,8 +722,8 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
         for (int r = 0; r < PF; ++r) {
             pf[r].x = (GR)0;
             pf[r].y = (GR)0;
-            if (pc[r] < 16) {
-                const int c = pc[r], p = pp_[r];
+            if ((pcp[r] >> 16) < 16) {
+                const int c = pcp[r] >> 16, p = pcp[r] & 0xffff;
                 const int e = g_ * nsub + c / tb, t = cb_ + c % tb;
                 if (e < E && t < T.B) {
                     int m = it.mw0 + p;
@@ -732,7 +744,7 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
             __syncthreads();   // the previous group's window is consumed
 #pragma unroll
             for (int r = 0; r < PF; ++r)
-                if (pc[r] < 16) s_g[pc[r] * kIrStride + pp_[r]] = pf[r];
+                if ((pcp[r] >> 16) < 16) s_g[(pcp[r] >> 16) * kIrStride + (pcp[r] & 0xffff)] = pf[r];
             __syncthreads();
             if (g + 1 < G) load_group(cb, g + 1);
             else if (cb + tb < T.B) load_group(cb + tb, 0);
@@ -745,29 +757,29 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
                 const int t = wave + 4 * u;
                 if (t >= ntile) continue;
                 const C2 *row = s_g + jb * kIrStride + woff[u] + ka;
-                V dr0, di0, dr1, di1;
+                V dr0, di0;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) { dr0[v] = (R)0; di0[v] = (R)0; dr1[v] = (R)0; di1[v] = (R)0; }
+                for (int v = 0; v < 4; ++v) { dr0[v] = (R)0; di0[v] = (R)0; }
                 const C2 b0 = row[0], b1 = row[4], b2 = row[8];
                 dr0 = Mfma<R>::mma(phiA[u][0], (R)b0.x, dr0);
                 di0 = Mfma<R>::mma(phiA[u][0], (R)b0.y, di0);
-                dr1 = Mfma<R>::mma(phiA[u][1], (R)b1.x, dr1);
-                di1 = Mfma<R>::mma(phiA[u][1], (R)b1.y, di1);
+                dr0 = Mfma<R>::mma(phiA[u][1], (R)b1.x, dr0);
+                di0 = Mfma<R>::mma(phiA[u][1], (R)b1.y, di0);
                 if (nst[u] > 2) {
                     dr0 = Mfma<R>::mma(phiA[u][2], (R)b2.x, dr0);
                     di0 = Mfma<R>::mma(phiA[u][2], (R)b2.y, di0);
                 }
                 if (nst[u] > 3) {
                     const C2 b3 = row[12];
-                    dr1 = Mfma<R>::mma(phiA[u][3], (R)b3.x, dr1);
-                    di1 = Mfma<R>::mma(phiA[u][3], (R)b3.y, di1);
+                    dr0 = Mfma<R>::mma(phiA[u][3], (R)b3.x, dr0);
+                    di0 = Mfma<R>::mma(phiA[u][3], (R)b3.y, di0);
                 }
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int i = 16 * t + Mfma<R>::row(lane, v);
                     const double pp = eok ? s_P[i][ec] : 0.0;
-                    const double gr = (double)dr0[v] + (double)dr1[v];
-                    const double gi = (double)di0[v] + (double)di1[v];
+                    const double gr = (double)dr0[v];
+                    const double gi = (double)di0[v];
                     // S += p i^(e mod 2) G
                     Sr[u][v] = fma(pp, odd ? -gi : gr, Sr[u][v]);
                     Si[u][v] = fma(pp, odd ? gr : gi, Si[u][v]);
