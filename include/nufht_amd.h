@@ -160,6 +160,9 @@ typedef struct {
      * (source, channel) and (target, channel) pairs (not in the spread_sc_*
      * and interp_tc_* counts) */
     long long fused_boxes, fused_sc, fused_tc;
+    /* local-expansion evaluation items that loop over every box covering
+     * their targets (one atomic add per output element per item) */
+    long long loc_eval_multi;
 } NufhtPlanStats;
 int nufht_amd_plan_stats(const NufhtPlan *plan, int batch, NufhtPlanStats *st);
 /* The same statistics summed over the transforms of a multi-transform batch

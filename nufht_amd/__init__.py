@@ -85,7 +85,7 @@ class PlanStats(ctypes.Structure):
             "spread_sc_valu", "spread_sc_mfma", "interp_tc_valu", "interp_tc_mfma")] + \
         [(n, ctypes.c_double) for n in ("asy_fine", "asy_fine_nlogn", "fft_bytes", "model_f64",
                                         "model_f32", "model_bytes", "model_dir", "model_loc")] + \
-        [(n, ctypes.c_longlong) for n in ("fused_boxes", "fused_sc", "fused_tc")]
+        [(n, ctypes.c_longlong) for n in ("fused_boxes", "fused_sc", "fused_tc", "loc_eval_multi")]
 
 
 class Knobs(ctypes.Structure):

@@ -415,6 +415,7 @@ static void free_program(Program *P)
     hipFree(P->d_tfs); hipFree(P->d_zero); hipFree(P->d_dir); hipFree(P->d_dir_rng);
     hipFree(P->d_lbox);
     hipFree(P->d_lproj); hipFree(P->d_leval); hipFree(P->d_leval1); hipFree(P->d_lev_boxes);
+    hipFree(P->d_leval1h); hipFree(P->d_lfold);
     hipFree(P->d_coef);
     hipFree(P->d_abox);
     hipFree(P->d_aspread); hipFree(P->d_ainterp); hipFree(P->d_grid); hipFree(P->d_ft);
@@ -640,9 +641,12 @@ static void add_loc_eval_items(Program *P)
                 }
         // atomics saved per target ~ B (boxes per covered target - 1)
         if (covered == 0 || (double)B * ((double)cover / (double)covered - 1.0) < 40.0) {
+            // (half-integer orders: their own launch, 256 targets per item, no LDS)
+            const bool half = P->tfs[tf].half != 0;
+            const int tpb = half ? 256 : P->loc_tpb;
             for (int b : bl)
-                for (int s = 0; s < P->lbox[b].ni; s += P->loc_tpb)
-                    P->leval1.push_back({b, s, std::min(P->loc_tpb, P->lbox[b].ni - s), 0});
+                for (int s = 0; s < P->lbox[b].ni; s += tpb)
+                    (half ? P->leval1h : P->leval1).push_back({b, s, std::min(tpb, P->lbox[b].ni - s), 0});
             continue;
         }
         for (size_t q = 0; q + 1 < cuts.size(); ++q) {
@@ -970,7 +974,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
         add_dir_items(P, tf, B, dirb);
         if (!locb.empty()) {
             kloc_max = std::max(kloc_max, T.K_loc);
-            slots_max = std::max(slots_max, loc_slots(T.order, T.half, T.K_loc));
+            slots_max = std::max(slots_max, loc_lds_slots(T.order, T.half, T.K_loc));
             if (T.half) {
                 int K = T.K_loc, N = 2 * K + 2, p = T.order & 1;
                 T.cheb_off = (int)cheb_all.size();
@@ -1305,8 +1309,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
             const TfDesc &T = P->tfs[P->lbox[b].tf];
             add_chunks_2d(P->lproj, (int)b, P->lbox[b].nj, P->loc_ch, T.B, kLocBM);
             if (T.B > 16) P->loc_bmax = 32;
+            if (T.half) P->lfold.push_back((int)b);
         }
         add_loc_eval_items(P);
+        P->stats.loc_eval_multi = (long long)P->leval.size();
     }
     if (!P->tfs.empty()) {
         P->stats.kernel_width = P->tfs[0].kw;
@@ -1340,6 +1346,8 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     RC_TRY(upload(&P->d_lproj, P->lproj));
     RC_TRY(upload(&P->d_leval, P->leval));
     RC_TRY(upload(&P->d_leval1, P->leval1));
+    RC_TRY(upload(&P->d_leval1h, P->leval1h));
+    RC_TRY(upload(&P->d_lfold, P->lfold));
     RC_TRY(upload(&P->d_lev_boxes, P->lev_boxes));
     RC_TRY(upload(&P->d_abox, P->abox));
     RC_TRY(upload(&P->d_aspread, P->aspread));
@@ -1551,6 +1559,9 @@ static int run_near_field(Program *P, hipStream_t st, double *coef)
         {
             StageTimer tm(ST_LOCP, st);
             launch_loc_proj(P, st, coef);
+            if (!P->lfold.empty())
+                hipLaunchKernelGGL(k_loc_fold, dim3((unsigned)P->lfold.size()), dim3(256), 0, st,
+                                   P->d_tfs, P->d_lbox, P->d_lfold, coef, P->d_cheb);
         }
         {
             StageTimer tm(ST_LOCE, st);
@@ -1563,6 +1574,16 @@ static int run_near_field(Program *P, hipStream_t st, double *coef)
                     hipLaunchKernelGGL(k_loc_eval1<double>, dim3((unsigned)P->leval1.size()),
                                        dim3(P->loc_tpb), P->loc_lds_e, st, P->d_tfs, P->d_lbox,
                                        P->d_leval1, coef, P->d_cheb);
+            }
+            if (!P->leval1h.empty()) {
+                if (P->prec == 2)
+                    hipLaunchKernelGGL(k_loc_eval1<float>, dim3((unsigned)P->leval1h.size()),
+                                       dim3(256), 0, st, P->d_tfs, P->d_lbox, P->d_leval1h, coef,
+                                       P->d_cheb);
+                else
+                    hipLaunchKernelGGL(k_loc_eval1<double>, dim3((unsigned)P->leval1h.size()),
+                                       dim3(256), 0, st, P->d_tfs, P->d_lbox, P->d_leval1h, coef,
+                                       P->d_cheb);
             }
             if (!P->leval.empty()) {
                 const bool f32 = P->prec == 2;

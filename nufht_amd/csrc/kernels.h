@@ -334,47 +334,29 @@ __global__ __launch_bounds__(256) void k_loc_proj(const TfDesc *__restrict__ tfs
 // Local expansion, evaluation half (reference src/expansions.c:275-336):
 // integer orders compute J_0..J_{l0+K+odd}(z), z = w r_end / 2, by one
 // Miller sweep and form b_l = J_{l0+odd+l}(z) J_{|l0-l|}(z) (sign (-1) for
-// l > l0 with l - l0 odd); half-integer orders sample j_n(w r_end x_q) at
-// the K+1 positive Chebyshev nodes and map them to coefficients with the
-// precomputed matrix cheb[].  Then g[i,t] += sum_l b_l coef[l][t] for every
-// column t.  One thread per target.  k_loc_eval1: one local box per item;
-// the sums go out four columns at a time.  k_loc_eval: every box that
-// covers the item's targets (about 2.6 at the clustered workload), looped
-// over with up to BMAX column sums in registers, so that each output element
-// gets one atomic add per item instead of one per box (the registers cost
-// occupancy, so transforms whose targets mostly meet one box keep the first
-// form).  The per-target scratch lives in LDS as
-// [slot][thread] (slots = loc_slots()), so lanes touch consecutive words;
-// a box's coefficients are the same for the whole block and are read as
-// uniform loads.
-// b_l = sum_q M[l][q] f_q for l <= K, f_q at slots K + 1 + q, b_l to slots
-// l: four rows at a time (independent accumulators, one read of f_q each)
-__device__ __forceinline__ void cheb_apply(double *S, int nthr, int K, const double *__restrict__ M)
-{
-    const int n = K + 1;
-    for (int l = 0; l <= K; l += 4) {
-        const double *m0 = M + (long long)l * n;
-        const double *m1 = l + 1 <= K ? m0 + n : m0;
-        const double *m2 = l + 2 <= K ? m0 + 2 * n : m0;
-        const double *m3 = l + 3 <= K ? m0 + 3 * n : m0;
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        for (int q = 0; q <= K; ++q) {
-            const double f = S[(n + q) * nthr];
-            a0 = fma(m0[q], f, a0);
-            a1 = fma(m1[q], f, a1);
-            a2 = fma(m2[q], f, a2);
-            a3 = fma(m3[q], f, a3);
-        }
-        S[l * nthr] = a0;
-        if (l + 1 <= K) S[(l + 1) * nthr] = a1;
-        if (l + 2 <= K) S[(l + 2) * nthr] = a2;
-        if (l + 3 <= K) S[(l + 3) * nthr] = a3;
-    }
-}
-
+// l > l0 with l - l0 odd), then g[i,t] += sum_l b_l coef[l][t] for every
+// column t; half-integer orders sample j_n(w r_end x_q) at the K+1 positive
+// Chebyshev nodes, whose map to coefficients (the precomputed matrix
+// cheb[]) is folded into the box's moments beforehand (k_loc_fold), and
+// contract the samples with them.  One thread per target.  k_loc_eval1: one
+// local box per item; the sums go out four (integer) or eight (half-integer)
+// columns at a time.  k_loc_eval: every box that covers the item's targets
+// (about 2.6 at the clustered workload), looped over with up to BMAX column
+// sums in registers, so that each output element gets one atomic add per
+// item instead of one per box (the registers cost occupancy, so transforms
+// whose targets mostly meet one box keep the first form).  The integer
+// orders' per-target scratch lives in LDS as [slot][thread] (slots =
+// loc_slots()), so lanes touch consecutive words; half-integer orders use
+// none.  A box's coefficients are the same for the whole block and are read
+// as uniform loads.
 __host__ __device__ inline int loc_slots(int order, int half, int K)
 {
     return half ? 2 * (K + 1) : (order >> 1) + K + (order & 1) + 1;
+}
+// LDS words per target the evaluation kernels use (none for half-integer orders)
+__host__ __device__ inline int loc_lds_slots(int order, int half, int K)
+{
+    return half ? 0 : loc_slots(order, half, K);
 }
 
 template <typename LR>
@@ -448,41 +430,68 @@ __global__ __launch_bounds__(256) void k_loc_eval1(const TfDesc *__restrict__ tf
         }
         return;
     }
-    double scale = 1.0;
-    int boff;               // slot of b_0 (b_l at slot boff + l)
-    {
-        // f_q = j_n(z x_q), x_q = cos(pi (q + 1/2) / (2K + 2)), q = 0..K, at
-        // slots K+1+q; b_l at slots l
-        const double z = w * bx.r_end;
-        const double *M = cheb + T.cheb_off;
-        const double *xn = M + (K + 1) * (K + 1);     // the nodes x_q, after the matrix
+    // half-integer orders: g_t = sign sqrt(2 w / pi) sum_q j_n(z x_q) C'[q][t]
+    // at the nodes x_q = cos(pi (q + 1/2) / (2K + 2)), q = 0..K, with the
+    // moments folded with the interpolation matrix (k_loc_fold: C' = M^T
+    // coef, so that the sum equals sum_l b_l coef[l][t] with b_l = sum_q
+    // M[l][q] j_n(z x_q)); no scratch, 8 columns per pass over the nodes
+    const double z = w * bx.r_end;
+    const double *xn = cheb + T.cheb_off + (K + 1) * (K + 1);   // the nodes, after the matrix
+    const double sc = sc0 * sqrt(2.0 * w / kPi);
+    const double *cf = coef + bx.coef_off;
+    for (int t0 = 0; t0 < B; t0 += 8) {
+        const int nt = min(8, B - t0);
+        LR a[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) a[t] = (LR)0;
         for (int q = 0; q <= K; ++q) {
             const double xq = xn[q];
-            double hi = z * xq, lo = fma(z, xq, -hi);
-            S[(K + 1 + q) * nthr] = sph_jn_cw(T.order, hi, lo);
-        }
-        cheb_apply(S, nthr, K, M);
-        boff = 0;
-        scale = sqrt(2.0 * w / kPi);
-    }
-    const double sc = sc0 * scale;
-    const double *cf = coef + bx.coef_off;
-    for (int t0 = 0; t0 < B; t0 += 4) {
-        const int nt = min(4, B - t0);
-        LR a0 = (LR)0, a1 = (LR)0, a2 = (LR)0, a3 = (LR)0;
-        for (int l = 0; l <= K; ++l) {
-            const LR bl = (LR)S[(boff + l) * nthr];
-            const double *cr = cf + (long long)l * B + t0;
-            a0 = fma(bl, (LR)cr[0], a0);
-            if (nt > 1) a1 = fma(bl, (LR)cr[1], a1);
-            if (nt > 2) a2 = fma(bl, (LR)cr[2], a2);
-            if (nt > 3) a3 = fma(bl, (LR)cr[3], a3);
+            const double hi = z * xq, lo = fma(z, xq, -hi);
+            const LR f = (LR)sph_jn_cw(T.order, hi, lo);
+            const double *cr = cf + (long long)q * B + t0;
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (t < nt) a[t] = fma(f, (LR)cr[t], a[t]);
         }
         double *g = T.gs + (long long)t0 * T.ld_gs + i;
-        atomicAdd(g, sc * (double)a0);
-        if (nt > 1) atomicAdd(g + T.ld_gs, sc * (double)a1);
-        if (nt > 2) atomicAdd(g + 2 * (long long)T.ld_gs, sc * (double)a2);
-        if (nt > 3) atomicAdd(g + 3 * (long long)T.ld_gs, sc * (double)a3);
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            if (t < nt) atomicAdd(g + (long long)t * T.ld_gs, sc * (double)a[t]);
+    }
+}
+
+// Half-integer orders: a local box's moments folded with the Chebyshev
+// interpolation matrix of its transform, in place,
+//   C'[q][t] = sum_l M[l][q] coef[l][t],  q, l = 0..K,
+// so that the evaluation contracts the node values j_n(z x_q) directly.
+// One block per box, kFoldCols columns at a time through LDS.
+constexpr int kFoldCols = 32;
+__global__ __launch_bounds__(256) void k_loc_fold(const TfDesc *__restrict__ tfs,
+                                                  const LocBoxD *__restrict__ boxes,
+                                                  const int *__restrict__ list,
+                                                  double *__restrict__ coef,
+                                                  const double *__restrict__ cheb)
+{
+    __shared__ double s_c[16 * kLocMaxNlt * kFoldCols];
+    const LocBoxD bx = boxes[list[blockIdx.x]];
+    const TfDesc &T = tfs[bx.tf];
+    const int n = T.K_loc + 1, B = T.B;
+    const double *M = cheb + T.cheb_off;
+    double *c = coef + bx.coef_off;
+    for (int t0 = 0; t0 < B; t0 += kFoldCols) {
+        const int nc = min(kFoldCols, B - t0);
+        __syncthreads();
+        for (int o = threadIdx.x; o < n * nc; o += 256) {
+            const int l = o / nc, t = o - l * nc;
+            s_c[l * kFoldCols + t] = c[(long long)l * B + t0 + t];
+        }
+        __syncthreads();
+        for (int o = threadIdx.x; o < n * nc; o += 256) {
+            const int q = o / nc, t = o - q * nc;
+            double a = 0.0;
+            for (int l = 0; l < n; ++l) a = fma(M[(long long)l * n + q], s_c[l * kFoldCols + t], a);
+            c[(long long)q * B + t0 + t] = a;
+        }
     }
 }
 
@@ -545,18 +554,21 @@ __global__ __launch_bounds__(256) void k_loc_eval(const TfDesc *__restrict__ tfs
             }
             boff = l0 + odd;
         } else {
-            // f_q = j_n(z x_q), x_q = cos(pi (q + 1/2) / (2K + 2)), q = 0..K, at
-            // slots K+1+q; b_l at slots l
+            // j_n(z x_q) at the nodes against the folded moments C'[q][t]
+            // (k_loc_fold), as in k_loc_eval1
             const double z = w * bx.r_end;
-            const double *M = cheb + T.cheb_off;
-            const double *xn = M + (K + 1) * (K + 1);   // the nodes x_q, after the matrix
+            const double *xn = cheb + T.cheb_off + (K + 1) * (K + 1);
+            const double *cf = coef + bx.coef_off + it.col0;
             for (int qq = 0; qq <= K; ++qq) {
                 const double xq = xn[qq];
-                double hi = z * xq, lo = fma(z, xq, -hi);
-                S[(K + 1 + qq) * nthr] = sph_jn_cw(T.order, hi, lo);
+                const double hi = z * xq, lo = fma(z, xq, -hi);
+                const LR f = (LR)sph_jn_cw(T.order, hi, lo);
+                const double *cr = cf + (long long)qq * B;
+#pragma unroll
+                for (int t = 0; t < BMAX; ++t)
+                    if (t < nb) acc[t] = fma(f, (LR)cr[t], acc[t]);
             }
-            cheb_apply(S, nthr, K, M);
-            boff = 0;
+            continue;
         }
         const double *cf = coef + bx.coef_off + it.col0;
         for (int l = 0; l <= K; ++l) {

@@ -111,6 +111,10 @@ struct Program {
     LocItem *d_lproj = nullptr;
     std::vector<LocItem> leval1;      // targets covered by one local box
     LocItem *d_leval1 = nullptr;
+    std::vector<LocItem> leval1h;     // the same for half-integer orders (no LDS scratch)
+    LocItem *d_leval1h = nullptr;
+    std::vector<int> lfold;           // half-integer local boxes, whose moments k_loc_fold folds
+    int *d_lfold = nullptr;
     std::vector<LocEvalItem> leval;   // targets covered by several
     LocEvalItem *d_leval = nullptr;
     std::vector<int> lev_boxes;       // box lists of the evaluation items

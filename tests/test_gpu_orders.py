@@ -113,3 +113,51 @@ def test_autograd_two_node_sets_in_one_graph(device):
     assert rt1.grad is not None and rt2.grad is not None
     assert np.all(np.isfinite(rt1.grad.cpu().numpy()))
     op.close()
+
+
+@pytest.mark.parametrize("nu,B,tol,prec", [
+    (1.5, 12, 1e-10, "fp64"),   # two passes of 8 columns over the Chebyshev nodes
+    (2.5, 3, 1e-6, "mixed"),
+    (0.5, 9, 1e-6, "mixed"),
+])
+def test_half_integer_local_expansions(device, nu, B, tol, prec):
+    """Half-integer orders: local boxes evaluate j_n at the Chebyshev nodes
+    against moments folded with the interpolation matrix."""
+    import torch
+    import nufht_amd as na
+    r, k, cs = _uniform(int(10 * nu) + B, 100000, 100000, 100.0, 100.0, B)
+    p = na.Plan(nu, r, k, tol=tol, precision=prec)
+    st = p.stats(B)
+    say(f"nu={nu} B={B} tol={tol} {prec} M=N=1e5 on [0,100]^2: K_loc={p.K_loc} "
+        f"boxes loc/asy/dir {st['n_loc']}/{st['n_asy']}/{st['n_dir']}")
+    assert st["n_loc"] > 0
+    g = p.forward(torch.as_tensor(np.ascontiguousarray(cs.T), device=device).t())
+    torch.cuda.synchronize()
+    rows = np.unique((W.Stream(int(10 * nu), 13).uniform(64) * k.size).astype(np.int32))
+    ref = oracle.direct_rows(nu, r, cs, k, rows)
+    err = rel_l2(g.cpu().numpy()[rows], ref)
+    say(f"  rel l2 {err:.2e}")
+    assert err < (2 * tol if prec == "fp64" else 1e-6)
+    p.close()
+
+
+def test_half_integer_multibox_local_evaluation(device):
+    """The clustered nodes at nu = 1.5 with 32 columns: local evaluation
+    items that loop over every box covering their targets, half-integer
+    path (checked to be in use)."""
+    import torch
+    import nufht_amd as na
+    wl = W.clustered()
+    r, k, cs = wl["r"], wl["k"], wl["cs"]
+    p = na.Plan(1.5, r, k, tol=1e-6, precision="fp64")
+    st = p.stats(cs.shape[1])
+    say(f"clustered nu=1.5 32 columns tol=1e-6: multi-box local items {st['loc_eval_multi']}")
+    assert st["loc_eval_multi"] > 0
+    g = p.forward(torch.as_tensor(np.asfortranarray(cs).T.copy(), device=device).t())
+    torch.cuda.synchronize()
+    rows = np.unique((W.Stream(15, 9).uniform(48) * k.size).astype(np.int32))
+    ref = oracle.direct_rows(1.5, r, cs, k, rows)
+    err = rel_l2(g.cpu().numpy()[rows], ref)
+    say(f"  rel l2 {err:.2e}")
+    assert err < 1e-6
+    p.close()
