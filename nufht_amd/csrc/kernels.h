@@ -344,19 +344,28 @@ __global__ __launch_bounds__(256) void k_loc_proj(const TfDesc *__restrict__ tfs
 // (about 2.6 at the clustered workload), looped over with up to BMAX column
 // sums in registers, so that each output element gets one atomic add per
 // item instead of one per box (the registers cost occupancy, so transforms
-// whose targets mostly meet one box keep the first form).  The integer
-// orders' per-target scratch lives in LDS as [slot][thread] (slots =
-// loc_slots()), so lanes touch consecutive words; half-integer orders use
-// none.  A box's coefficients are the same for the whole block and are read
-// as uniform loads.
+// whose targets mostly meet one box keep the first form).  The per-target
+// scratch lives in LDS as [slot][thread] (slots = loc_lds_slots(); none in
+// k_loc_eval1's half-integer path), so lanes touch consecutive words.  A
+// box's coefficients are the same for the whole block and are read as
+// uniform loads.
 __host__ __device__ inline int loc_slots(int order, int half, int K)
 {
     return half ? 2 * (K + 1) : (order >> 1) + K + (order & 1) + 1;
 }
-// LDS words per target the evaluation kernels use (none for half-integer orders)
+// LDS words per target the evaluation kernels use: for half-integer orders
+// the K + 1 node values (multi-box items; k_loc_eval1 keeps them in registers)
 __host__ __device__ inline int loc_lds_slots(int order, int half, int K)
 {
-    return half ? 0 : loc_slots(order, half, K);
+    return half ? K + 1 : loc_slots(order, half, K);
+}
+
+// j_n(hi + lo) out of line for k_loc_eval1's half-integer path (once per
+// Chebyshev node): inlined there it raised the kernel's register count, and
+// with it the integer orders' occupancy
+__device__ __noinline__ double sph_jn_node(int n, double hi, double lo)
+{
+    return sph_jn_cw(n, hi, lo);
 }
 
 template <typename LR>
@@ -447,7 +456,7 @@ __global__ __launch_bounds__(256) void k_loc_eval1(const TfDesc *__restrict__ tf
         for (int q = 0; q <= K; ++q) {
             const double xq = xn[q];
             const double hi = z * xq, lo = fma(z, xq, -hi);
-            const LR f = (LR)sph_jn_cw(T.order, hi, lo);
+            const LR f = (LR)sph_jn_node(T.order, hi, lo);
             const double *cr = cf + (long long)q * B + t0;
 #pragma unroll
             for (int t = 0; t < 8; ++t)
@@ -554,21 +563,16 @@ __global__ __launch_bounds__(256) void k_loc_eval(const TfDesc *__restrict__ tfs
             }
             boff = l0 + odd;
         } else {
-            // j_n(z x_q) at the nodes against the folded moments C'[q][t]
-            // (k_loc_fold), as in k_loc_eval1
+            // j_n(z x_q) at the nodes, slots q, contracted below with the
+            // folded moments C'[q][t] (k_loc_fold) as b_l with coef[l][t]
             const double z = w * bx.r_end;
             const double *xn = cheb + T.cheb_off + (K + 1) * (K + 1);
-            const double *cf = coef + bx.coef_off + it.col0;
             for (int qq = 0; qq <= K; ++qq) {
                 const double xq = xn[qq];
                 const double hi = z * xq, lo = fma(z, xq, -hi);
-                const LR f = (LR)sph_jn_cw(T.order, hi, lo);
-                const double *cr = cf + (long long)qq * B;
-#pragma unroll
-                for (int t = 0; t < BMAX; ++t)
-                    if (t < nb) acc[t] = fma(f, (LR)cr[t], acc[t]);
+                S[qq * nthr] = sph_jn_cw(T.order, hi, lo);
             }
-            continue;
+            boff = 0;
         }
         const double *cf = coef + bx.coef_off + it.col0;
         for (int l = 0; l <= K; ++l) {
