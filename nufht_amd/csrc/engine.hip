@@ -775,23 +775,26 @@ static void add_interp_items(Program *P, const std::vector<TfInput> &in)
                 continue;
             }
             flush_valu(s);
-            const int mw = m0f - 1;    // the tile's window: [mw, mw + 16)
+            // the tile's window: [mw, mw + 16) for the one-tile items; the run
+            // kernel starts it at the device's first tap, which may differ from
+            // the host's m0f by one either way, so runs stage [mw, mw + 18)
+            const int mw = m0f - 1;
             if (tiles) {
                 // one item per tile, every column chunk
                 P->aimt.push_back({box, s, cnt, 0, tb, mw});
                 P->stats.interp_tc_mfma += (long long)cnt * A.E * T.B;
                 continue;
             }
-            if (cur.count && (cur.count == 16 * kIrTiles || mw + 16 - cur.mw0 > kIrPos ||
+            if (cur.count && (cur.count == 16 * kIrTiles || mw + 18 - cur.mw0 > kIrPos ||
                               mw < cur.mw0))
                 flush_run();
             if (!cur.count) {
                 cur.start = s;
                 cur.mw0 = mw;
-                cur.npos = 16;
+                cur.npos = 18;
             }
             cur.count += cnt;
-            cur.npos = std::max(cur.npos, mw + 16 - cur.mw0);
+            cur.npos = std::max(cur.npos, mw + 18 - cur.mw0);
         }
         flush_run();
         flush_valu(A.ni);

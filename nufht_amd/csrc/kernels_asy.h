@@ -676,11 +676,11 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
 #pragma unroll
         for (int st = 0; st < 4; ++st) phiA[u][st] = (R)0;
         if (t < ntile) {
-            const int mw = s_m0[b0] - 1;                 // the tile's window start
+            const int mw = s_m0[b0];                     // the tile's window start
             woff[u] = mw - it.mw0;
             // K steps of 4 positions that hold a kernel tap of some target
-            // (the first target's taps start at mw + 1, the last's end at
-            // its m0 + kw - 1): the others are all zero and are skipped
+            // (the first target's taps start at mw, the last's end at its
+            // m0 + kw - 1): the others are all zero and are skipped
             const int nsteps = min(4, (s_m0[b0 + cnt - 1] + kw - mw + 3) >> 2);
             nst[u] = nsteps;
             // phi(s - m Delta) at tap k = m - m0 of the target (zero outside
