@@ -328,10 +328,28 @@ __global__ __launch_bounds__(kFftThreads, 4) void k_fft_cols(typename Cplx<R>::T
         if (p <= L || p >= nf - L) v = base[(long long)n * P.N2 + c];
         return v;
     };
+    // The last stage hands a thread's outputs k1 = j + r Ns, r = 0, 1, ...
+    // (Ns = N1 / its radix) in that order: in FP64 the twiddles w^{k1 n2}
+    // follow from w^{j n2} and w^{Ns n2} (two table products) by a running
+    // product, instead of two table loads per element; FP32 grids read the
+    // table for every element.
+    const int Ns = P.N / P.radix[P.nstage - 1];
+    C2 twc, tws;
     auto st = [&](int c, int k1, C2 v) {
-        const int a = k1 * (t * M + c);          // < (N1 - 1) (N2 - 1) < Nf
-        const C2 w = CMul<R>::mul(hi[a >> 8], lo[a & 255]);
-        base[(long long)k1 * P.N2 + c] = CMul<R>::mul(v, w);
+        const int n2 = t * M + c;
+        if constexpr (sizeof(R) == 8) {
+            if (k1 < Ns) {
+                const int a = k1 * n2, b = Ns * n2;   // < Ns N2 <= Nf
+                twc = CMul<R>::mul(hi[a >> 8], lo[a & 255]);
+                tws = CMul<R>::mul(hi[b >> 8], lo[b & 255]);
+            } else {
+                twc = CMul<R>::mul(twc, tws);
+            }
+        } else {
+            const int a = k1 * n2;                  // < (N1 - 1) (N2 - 1) < Nf
+            twc = CMul<R>::mul(hi[a >> 8], lo[a & 255]);
+        }
+        base[(long long)k1 * P.N2 + c] = CMul<R>::mul(v, twc);
     };
     fft_tile<R>(s, P, tw + P.w_off, ld, st);
 }
