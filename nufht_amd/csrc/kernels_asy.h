@@ -218,12 +218,15 @@ __global__ __launch_bounds__(256, 4) void k_asy_spread(const TfDesc *__restrict_
 // Row strides are padded so that both the staging writes (one source per
 // lane) and the product reads (4 sources x 4 positions or exponents per
 // instruction) spread over the LDS banks.
+constexpr int kSpreadDec = 128;   // deconvolution factors staged per chunk
 template <typename R, int EMAX> struct SpreadLds {
     typedef typename Cplx<R>::T C2;
     R psi[kSQ][kSP + 5];          // kernel value of source q at position l: column l mod 16
     C2 u[kSQ][EMAX + 1];          // e^{iDx} r^{-1/2} (rho/r)^e
     R bop[kSQ][17];               // A operand: c[q][t] (rho/r)^{e2}, row j = (t, e2)
+    double dec[kSpreadDec];       // 1 / (a2 F(l h a2)) for l = dec_base + i
     int lf[kSQ];                  // first kernel tap of source q
+    int dec_base;
 };
 
 // Products of one staged chunk: KC of the block's products, accumulated over
@@ -282,8 +285,8 @@ __device__ __forceinline__ void spread_step(const SpreadLds<R, EMAX> &L, int s, 
 
 // Global loads of one producer thread for one chunk (its source's r; for
 // waves 2-3 also 8 coefficients), issued a chunk ahead of their use.
-template <bool CV> struct SpreadIn { double r; double cv[8]; };
-template <> struct SpreadIn<false> { double r; };
+template <bool CV> struct SpreadIn { static constexpr bool kCV = true; double r; double cv[8]; };
+template <> struct SpreadIn<false> { static constexpr bool kCV = false; double r; };
 
 template <bool CV>
 __device__ __forceinline__ void spread_load(const TfDesc &T, const AsyItem &it, int c0, int cn,
@@ -305,7 +308,7 @@ __device__ __forceinline__ void spread_load(const TfDesc &T, const AsyItem &it, 
 // wave 0 the phases and powers, wave 1 the kernel values at the positions
 // the sources reach, waves 2-3 the A operand.  Rows past the chunk's end
 // are zero.
-template <typename R, int EMAX, bool CV>
+template <typename R, int EMAX, bool CV, bool DEC>
 __device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx, const AsyItem &it,
                                              int c0, int cn, int pw, int ptid,
                                              const SpreadIn<CV> &in, const double *__restrict__ ft,
@@ -364,6 +367,16 @@ __device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx,
             }
             L.bop[q][j] = (R)c;
         }
+        if constexpr (DEC) {
+            // the deconvolution factors of positions dec_base .. dec_base +
+            // 127, from 32 below the chunk's first tap: the window flushes
+            // while the chunk is multiplied reach mostly those (others
+            // evaluate their own)
+            const double r0 = __shfl(in.r, 0, 64);
+            const int base = (int)ceil((r0 - bx.C - bx.a1) / bx.h) - 32, i = ptid - 128;
+            L.dec[i] = 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)(base + i) * bx.h * bx.a2, T.kmax));
+            if (i == 0) L.dec_base = base;
+        }
     }
 }
 
@@ -371,7 +384,7 @@ __device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx,
 // while the consumers multiply chunk k, one block barrier per chunk (as many
 // as the consumers pass).  The loads of chunk k + 1 are in flight while
 // chunk k is staged and the barrier waits.
-template <typename R, int EMAX, bool CV>
+template <typename R, int EMAX, bool CV, bool DEC>
 __device__ __forceinline__ void spread_produce(const TfDesc &T, const AsyBoxD &bx,
                                                const AsyItem &it, int2 piece, int nchunk, int pw,
                                                int ptid, const double *__restrict__ ft,
@@ -380,7 +393,7 @@ __device__ __forceinline__ void spread_produce(const TfDesc &T, const AsyBoxD &b
     SpreadIn<CV> in;
     if (nchunk > 0) {
         spread_load(T, it, piece.x, min(kSQ, piece.y - piece.x), ptid, in);
-        spread_stage<R, EMAX>(T, bx, it, piece.x, min(kSQ, piece.y - piece.x), pw, ptid, in, ft,
+        spread_stage<R, EMAX, CV, DEC>(T, bx, it, piece.x, min(kSQ, piece.y - piece.x), pw, ptid, in, ft,
                               L[0]);
         if (nchunk > 1) {
             const int c1 = piece.x + kSQ;
@@ -393,7 +406,7 @@ __device__ __forceinline__ void spread_produce(const TfDesc &T, const AsyBoxD &b
             SpreadIn<CV> nx = in;
             const int c1 = piece.x + (k + 1) * kSQ, c2 = c1 + kSQ;
             if (k + 2 < nchunk) spread_load(T, it, c2, min(kSQ, piece.y - c2), ptid, nx);
-            spread_stage<R, EMAX>(T, bx, it, c1, min(kSQ, piece.y - c1), pw, ptid, in, ft,
+            spread_stage<R, EMAX, CV, DEC>(T, bx, it, c1, min(kSQ, piece.y - c1), pw, ptid, in, ft,
                                   L[(k + 1) & 1]);
             in = nx;
         }
@@ -424,14 +437,17 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
                                                  SpreadLds<R, EMAX> *L)
 {
     typedef typename Mfma<R>::V V;
+    // with one or two products per wave the window flushes weigh more than
+    // the products: their deconvolution factors are staged per chunk
+    constexpr bool DEC = KW <= 2;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int2 piece = rg.piece;
     const int nchunk = (piece.y - piece.x + kSQ - 1) / kSQ;
     if constexpr (SPLIT) {
         if (wv >= 4) {
             // waves 6-7 hold the coefficients they load, waves 4-5 only r
-            if (wv >= 6) spread_produce<R, EMAX, true>(T, bx, it, piece, nchunk, wv - 4, tid - 256, ft, L);
-            else spread_produce<R, EMAX, false>(T, bx, it, piece, nchunk, wv - 4, tid - 256, ft, L);
+            if (wv >= 6) spread_produce<R, EMAX, true, DEC>(T, bx, it, piece, nchunk, wv - 4, tid - 256, ft, L);
+            else spread_produce<R, EMAX, false, DEC>(T, bx, it, piece, nchunk, wv - 4, tid - 256, ft, L);
             return;
         }
         __syncthreads();
@@ -459,6 +475,7 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
         rowpk[v] = it.col0 + tt < T.B ? ((j / tb) << 8) | tt : -1;
     }
     int W = 0;
+    const SpreadLds<R, EMAX> *Lc = L;     // the chunk being multiplied (its staged factors)
     // positions [W, Wn) are final: write this lane's column if it holds one
     auto flush = [&](int Wn) {
         const int P = W + ((col - W) & 15);
@@ -475,7 +492,10 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
             }
             const bool excl = P >= rg.ex.x && P <= rg.ex.y;
             const int pos = P >= 0 ? P : P + bx.Nf;
-            const double dec = 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)P * bx.h * bx.a2, T.kmax));
+            const int di = DEC ? P - Lc->dec_base : -1;
+            const double dec = di >= 0 && di < kSpreadDec
+                                   ? Lc->dec[di]
+                                   : 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)P * bx.h * bx.a2, T.kmax));
 #pragma unroll
             for (int k = 0; k < KW; ++k) {
                 if (k >= mc) break;
@@ -509,15 +529,16 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
             if (wv >= 2) {
                 SpreadIn<true> in;
                 spread_load(T, it, c0, cn, tid, in);
-                spread_stage<R, EMAX>(T, bx, it, c0, cn, wv, tid, in, ft, L[0]);
+                spread_stage<R, EMAX, decltype(in)::kCV, DEC>(T, bx, it, c0, cn, wv, tid, in, ft, L[0]);
             } else {
                 SpreadIn<false> in;
                 spread_load(T, it, c0, cn, tid, in);
-                spread_stage<R, EMAX>(T, bx, it, c0, cn, wv, tid, in, ft, L[0]);
+                spread_stage<R, EMAX, decltype(in)::kCV, DEC>(T, bx, it, c0, cn, wv, tid, in, ft, L[0]);
             }
             __syncthreads();
         }
         const SpreadLds<R, EMAX> &Lk = L[SPLIT ? (k & 1) : 0];
+        Lc = &Lk;
         const int l0 = __builtin_amdgcn_readfirstlane(Lk.lf[0]);
         const int l1 = __builtin_amdgcn_readfirstlane(Lk.lf[cn - 1]);
         if (k == 0) W = l0 + kw - 15;
