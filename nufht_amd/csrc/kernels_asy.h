@@ -539,8 +539,12 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
         }
         const SpreadLds<R, EMAX> &Lk = L[SPLIT ? (k & 1) : 0];
         Lc = &Lk;
-        const int l0 = __builtin_amdgcn_readfirstlane(Lk.lf[0]);
-        const int l1 = __builtin_amdgcn_readfirstlane(Lk.lf[cn - 1]);
+        // the chunk's first taps in a register (lane q holds source q's),
+        // read by lane index below instead of one LDS round trip each
+        const int lfl = Lk.lf[min(lane, cn - 1)];
+        auto lfq = [&](int q) { return __builtin_amdgcn_readlane(lfl, min(q, cn - 1)); };
+        const int l0 = lfq(0);
+        const int l1 = lfq(cn - 1);
         if (k == 0) W = l0 + kw - 15;
         if (l1 - l0 <= 15 - kw) {
             // the whole chunk fits one window
@@ -555,12 +559,10 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
             for (int s = 0; s < cn; s += 4) {
                 int lo = 0;
                 while (lo < 4) {
-                    const int llo = __builtin_amdgcn_readfirstlane(Lk.lf[min(s + lo, cn - 1)]);
+                    const int llo = lfq(s + lo);
                     advance(llo + kw - 15);
                     int hi = lo + 1;
-                    while (hi < 4 &&
-                           __builtin_amdgcn_readfirstlane(Lk.lf[min(s + hi, cn - 1)]) + kw <= W + 15)
-                        ++hi;
+                    while (hi < 4 && lfq(s + hi) + kw <= W + 15) ++hi;
                     if (mc == KW) spread_step<R, EMAX, KW, KW>(Lk, s, lo, hi, lane, uoff, acc);
                     else if constexpr (KW > 1) {
                         if (mc == KW - 1) spread_step<R, EMAX, KW - 1, KW>(Lk, s, lo, hi, lane, uoff, acc);
