@@ -268,21 +268,6 @@ __device__ __forceinline__ void spread_chunk(const SpreadLds<R, EMAX> &L, int qh
     }
 }
 
-// One step (sources s .. s + 3) restricted to the sources s + lo .. s + hi - 1.
-template <typename R, int EMAX, int KC, int KA>
-__device__ __forceinline__ void spread_step(const SpreadLds<R, EMAX> &L, int s, int lo, int hi,
-                                            int lane, const int (&uoff)[KA],
-                                            typename Mfma<R>::V (&acc)[KA])
-{
-    const int ka = lane >> 4, ja = lane & 15;
-    const int q = s + ka;
-    const R a = L.bop[q][ja];
-    const R ps = (ka >= lo && ka < hi) ? L.psi[q][ja] : (R)0;
-    const R *uq = (const R *)&L.u[q][0];
-#pragma unroll
-    for (int k = 0; k < KC; ++k) acc[k] = Mfma<R>::mma(a, ps * uq[uoff[k]], acc[k]);
-}
-
 // Global loads of one producer thread for one chunk (its source's r; for
 // waves 2-3 also 8 coefficients), issued a chunk ahead of their use.
 template <bool CV> struct SpreadIn { static constexpr bool kCV = true; double r; double cv[8]; };
@@ -556,17 +541,26 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
         } else {
             // step by step; a step whose sources do not fit one window runs
             // in parts, each after the window has moved for its first source
+            // (a step's operands are read before the window moves, so that
+            // the reads overlap the flush)
+            const int ka = lane >> 4, ja = lane & 15;
             for (int s = 0; s < cn; s += 4) {
+                const int q = s + ka;
+                const R a = Lk.bop[q][ja], pr = Lk.psi[q][ja];
+                const R *uq = (const R *)&Lk.u[q][0];
+                R uv[KW];
+#pragma unroll
+                for (int k = 0; k < KW; ++k) uv[k] = uq[uoff[k]];
                 int lo = 0;
                 while (lo < 4) {
                     const int llo = lfq(s + lo);
                     advance(llo + kw - 15);
                     int hi = lo + 1;
                     while (hi < 4 && lfq(s + hi) + kw <= W + 15) ++hi;
-                    if (mc == KW) spread_step<R, EMAX, KW, KW>(Lk, s, lo, hi, lane, uoff, acc);
-                    else if constexpr (KW > 1) {
-                        if (mc == KW - 1) spread_step<R, EMAX, KW - 1, KW>(Lk, s, lo, hi, lane, uoff, acc);
-                    }
+                    const R ps = (ka >= lo && ka < hi) ? pr : (R)0;
+#pragma unroll
+                    for (int k = 0; k < KW; ++k)
+                        if (k < mc) acc[k] = Mfma<R>::mma(a, ps * uv[k], acc[k]);
                     lo = hi;
                 }
             }
