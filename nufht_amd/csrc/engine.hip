@@ -260,6 +260,12 @@ static int g_fused_max = [] {
 int nufht_dev::knobs_default_fused_max() { return g_fused_max; }
 
 
+// matrix-core spread: source density below which items stage their
+// chunks' deconvolution factors (AsyItem::pad; measured on the uneven batch:
+// spread 8.85 -> 8.18 ms per step at 16 to 64, the other workloads unchanged
+// below 64 and slower when every item stages them)
+constexpr double kDecDensity = 32.0;
+
 // Two-pass factorisation Nf = N1 N2 for the FFT kernels (Nf = 2^a, 3 2^a or
 // 9 2^a above 4096; the factor 3 or 9 goes into N1), sequences per block and
 // radices.
@@ -1196,6 +1202,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                     return (int)std::ceil((h->r[j] - A.C - A.a1) / A.h);
                 };
                 const int split = dense_split;
+                // boxes with fewer than kDecDensity sources per grid position
+                // move their window often: their items stage the
+                // deconvolution factors of each chunk for the flushes
+                const int stage_dec = (double)A.nj < kDecDensity * (2.0 * A.L + 1.0) ? 1 : 0;
                 const int npiece = (A.nj + split - 1) / split;
                 for (int pk = 0; pk < npiece; ++pk) {
                     int j0 = A.j0 + pk * split, j1 = std::min(A.j0 + A.nj, j0 + split);
@@ -1206,7 +1216,7 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                     sr.ex = make_int2(pk > 0 ? lf_of(j0 - 1) + T.kw + 2 : sr.wr.x,
                                       pk + 1 < npiece ? lf_of(j1) - 2 : sr.wr.y);
                     sr.pad = make_int2(0, 0);
-                    P->adense.push_back({box, 0, 0, c0, tb, 0});
+                    P->adense.push_back({box, 0, 0, c0, tb, stage_dec});
                     P->adrange.push_back(sr);
                 }
             } else {

@@ -293,7 +293,7 @@ __device__ __forceinline__ void spread_load(const TfDesc &T, const AsyItem &it, 
 // wave 0 the phases and powers, wave 1 the kernel values at the positions
 // the sources reach, waves 2-3 the A operand.  Rows past the chunk's end
 // are zero.
-template <typename R, int EMAX, bool CV, bool DEC>
+template <typename R, int EMAX, bool CV>
 __device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx, const AsyItem &it,
                                              int c0, int cn, int pw, int ptid,
                                              const SpreadIn<CV> &in, const double *__restrict__ ft,
@@ -352,7 +352,7 @@ __device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx,
             }
             L.bop[q][j] = (R)c;
         }
-        if constexpr (DEC) {
+        if (it.pad) {
             // the deconvolution factors of positions dec_base .. dec_base +
             // 127, from 32 below the chunk's first tap: the window flushes
             // while the chunk is multiplied reach mostly those (others
@@ -369,7 +369,7 @@ __device__ __forceinline__ void spread_stage(const TfDesc &T, const AsyBoxD &bx,
 // while the consumers multiply chunk k, one block barrier per chunk (as many
 // as the consumers pass).  The loads of chunk k + 1 are in flight while
 // chunk k is staged and the barrier waits.
-template <typename R, int EMAX, bool CV, bool DEC>
+template <typename R, int EMAX, bool CV>
 __device__ __forceinline__ void spread_produce(const TfDesc &T, const AsyBoxD &bx,
                                                const AsyItem &it, int2 piece, int nchunk, int pw,
                                                int ptid, const double *__restrict__ ft,
@@ -378,7 +378,7 @@ __device__ __forceinline__ void spread_produce(const TfDesc &T, const AsyBoxD &b
     SpreadIn<CV> in;
     if (nchunk > 0) {
         spread_load(T, it, piece.x, min(kSQ, piece.y - piece.x), ptid, in);
-        spread_stage<R, EMAX, CV, DEC>(T, bx, it, piece.x, min(kSQ, piece.y - piece.x), pw, ptid, in, ft,
+        spread_stage<R, EMAX, CV>(T, bx, it, piece.x, min(kSQ, piece.y - piece.x), pw, ptid, in, ft,
                               L[0]);
         if (nchunk > 1) {
             const int c1 = piece.x + kSQ;
@@ -391,7 +391,7 @@ __device__ __forceinline__ void spread_produce(const TfDesc &T, const AsyBoxD &b
             SpreadIn<CV> nx = in;
             const int c1 = piece.x + (k + 1) * kSQ, c2 = c1 + kSQ;
             if (k + 2 < nchunk) spread_load(T, it, c2, min(kSQ, piece.y - c2), ptid, nx);
-            spread_stage<R, EMAX, CV, DEC>(T, bx, it, c1, min(kSQ, piece.y - c1), pw, ptid, in, ft,
+            spread_stage<R, EMAX, CV>(T, bx, it, c1, min(kSQ, piece.y - c1), pw, ptid, in, ft,
                                   L[(k + 1) & 1]);
             in = nx;
         }
@@ -422,17 +422,14 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
                                                  SpreadLds<R, EMAX> *L)
 {
     typedef typename Mfma<R>::V V;
-    // with one or two products per wave the window flushes weigh more than
-    // the products: their deconvolution factors are staged per chunk
-    constexpr bool DEC = KW <= 2;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int2 piece = rg.piece;
     const int nchunk = (piece.y - piece.x + kSQ - 1) / kSQ;
     if constexpr (SPLIT) {
         if (wv >= 4) {
             // waves 6-7 hold the coefficients they load, waves 4-5 only r
-            if (wv >= 6) spread_produce<R, EMAX, true, DEC>(T, bx, it, piece, nchunk, wv - 4, tid - 256, ft, L);
-            else spread_produce<R, EMAX, false, DEC>(T, bx, it, piece, nchunk, wv - 4, tid - 256, ft, L);
+            if (wv >= 6) spread_produce<R, EMAX, true>(T, bx, it, piece, nchunk, wv - 4, tid - 256, ft, L);
+            else spread_produce<R, EMAX, false>(T, bx, it, piece, nchunk, wv - 4, tid - 256, ft, L);
             return;
         }
         __syncthreads();
@@ -477,7 +474,7 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
             }
             const bool excl = P >= rg.ex.x && P <= rg.ex.y;
             const int pos = P >= 0 ? P : P + bx.Nf;
-            const int di = DEC ? P - Lc->dec_base : -1;
+            const int di = it.pad ? P - Lc->dec_base : -1;
             const double dec = di >= 0 && di < kSpreadDec
                                    ? Lc->dec[di]
                                    : 1.0 / (bx.a2 * ft_eval(ft + T.ft_off, (double)P * bx.h * bx.a2, T.kmax));
@@ -514,11 +511,11 @@ __device__ __forceinline__ void spread_mfma_body(const TfDesc &T, const AsyBoxD 
             if (wv >= 2) {
                 SpreadIn<true> in;
                 spread_load(T, it, c0, cn, tid, in);
-                spread_stage<R, EMAX, decltype(in)::kCV, DEC>(T, bx, it, c0, cn, wv, tid, in, ft, L[0]);
+                spread_stage<R, EMAX, decltype(in)::kCV>(T, bx, it, c0, cn, wv, tid, in, ft, L[0]);
             } else {
                 SpreadIn<false> in;
                 spread_load(T, it, c0, cn, tid, in);
-                spread_stage<R, EMAX, decltype(in)::kCV, DEC>(T, bx, it, c0, cn, wv, tid, in, ft, L[0]);
+                spread_stage<R, EMAX, decltype(in)::kCV>(T, bx, it, c0, cn, wv, tid, in, ft, L[0]);
             }
             __syncthreads();
         }
