@@ -210,6 +210,15 @@ __device__ __forceinline__ float es_kernel_f(float u, float beta)
     return t > 0.0f ? __expf(beta * (__fsqrt_rn(t) - 1.0f)) : 0.0f;
 }
 
+// Orders one wave's LDS writes before its later LDS reads (and reads before
+// later writes) where only that wave touches the data: no block barrier.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 constexpr int kVQ = 64;            // sources staged per LDS pass of the VALU spread
 // Exponents per lane of the VALU spread (4 EG accumulators: two columns,
 // complex): channels are split over lanes, so a lane's registers do not grow

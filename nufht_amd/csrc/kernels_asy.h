@@ -815,12 +815,13 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
             for (int off = tb; off < 16; off <<= 1)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) out[v] += __shfl_xor(out[v], off, 64);
-            __syncthreads();
+            // (s_out[wave] is this wave's own: wave-level ordering suffices)
+            wave_lds_sync();
             if (e2b == 0) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) s_out[wave][Mfma<R>::row(lane, v)][jb] = out[v];
             }
-            __syncthreads();
+            wave_lds_sync();
             if (t < ntile) {
                 for (int idx = lane; idx < 16 * tb; idx += 64) {
                     const int c = idx >> 4, i = idx & 15;
