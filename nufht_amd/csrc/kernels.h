@@ -222,8 +222,9 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
 // The contraction is a (K+1) x count x 16 matrix product per item, run on
 // the FP64 matrix cores: A[l][q] = B_l(x_q) (NLT row tiles of 16), B[q][t] =
 // c_q,t, accumulated in registers over every source of the item; a pass
-// stages kLocSub sources (basis by the three-term recurrence, one source per
-// lane; coefficients loaded column by column, coalesced) in LDS with odd row
+// stages kLocSub sources (basis by a three-term recurrence in steps of two
+// degrees, one source per lane; coefficients loaded column by column,
+// coalesced) in LDS with odd row
 // strides, the 4 waves take interleaved K steps of 4 sources, and the 4
 // partial tiles are summed through LDS before one atomic add per output.
 constexpr int kLocBM = 16;     // columns per item
@@ -270,20 +271,17 @@ __global__ __launch_bounds__(256) void k_loc_proj(const TfDesc *__restrict__ tfs
             if (tid < n) {
                 double x = T.r[bx.j0 + it.start + s0 + tid] / bx.r_end;
                 x = fmin(1.0, fmax(-1.0, x));
-                double tm1 = 1.0, t0 = x;   // T_0, T_1
+                // a_l = T_{2l+par}(x) by a_{l+1} = 2 T_2 a_l - a_{l-1} (from
+                // T_m T_n = (T_{m+n} + T_{|m-n|}) / 2), a_{-1} = T_{par-2} =
+                // T_{2-par}: one step per entry
+                const double ty = 2.0 * (2.0 * x * x - 1.0);
+                double am1 = par ? x : 0.5 * ty, a0 = par ? x : 1.0;
                 row[0] = (LR)(par ? 2.0 * x : 1.0);
-                int deg = 1;
                 for (int l = 1; l < 16 * NLT; ++l) {
-                    if (l <= K) {
-                        for (int target = 2 * l + par; deg < target; ++deg) {
-                            double tn = 2.0 * x * t0 - tm1;
-                            tm1 = t0;
-                            t0 = tn;
-                        }
-                        row[l] = (LR)(2.0 * t0);
-                    } else {
-                        row[l] = (LR)0;
-                    }
+                    const double a1 = fma(ty, a0, -am1);
+                    am1 = a0;
+                    a0 = a1;
+                    row[l] = l <= K ? (LR)(2.0 * a0) : (LR)0;
                 }
             } else {
                 for (int l = 0; l < 16 * NLT; ++l) row[l] = (LR)0;
