@@ -1368,6 +1368,13 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
     RC_TRY(upload(&P->d_azero, P->azero));
     RC_TRY(upload(&P->d_arange, P->arange));
     RC_TRY(upload(&P->d_adense, P->adense));
+    if (g_trace && !P->aimf.empty()) {
+        long long hist[kIrTiles + 1] = {0};
+        for (auto &r : P->aimf) ++hist[(r.count + 15) / 16];
+        fprintf(stderr, "[nufht_amd] interpolation runs %zu, by tiles:", P->aimf.size());
+        for (int t = 1; t <= kIrTiles; ++t) fprintf(stderr, " %lld", hist[t]);
+        fprintf(stderr, "\n");
+    }
     RC_TRY(upload(&P->d_aimf, P->aimf));
     RC_TRY(upload(&P->d_aimt, P->aimt));
     RC_TRY(upload(&P->d_adrange, P->adrange));
