@@ -62,6 +62,7 @@ struct AsyItem { int box, start, count, col0, tb, pad; };
 struct InterpRun { int box, start, count, mw0, npos, pad; };
 constexpr int kIrTiles = 8;    // 16-target tiles per matrix-core interpolation run
 constexpr int kIrPos = 64;     // grid positions a run stages, at most
+constexpr int kIrSteps = 5;    // K steps of 4 positions per tile: windows of up to 20
 struct ZeroItem { int tf, col, start, count; };
 
 constexpr int kNFT = 40;   // Chebyshev terms of the kernel transform table

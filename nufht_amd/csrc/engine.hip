@@ -775,7 +775,10 @@ static void add_interp_items(Program *P, const std::vector<TfInput> &in)
         for (int s = 0; s < A.ni; s += 16) {
             const int cnt = std::min(16, A.ni - s);
             const int m0f = m0_of(s), m0l = m0_of(s + cnt - 1);
-            if (!(P->knobs.interp_mfma && m0l - m0f + T.kw + 2 <= 16)) {
+            // window positions the tile's taps need, with one of slack: at
+            // most 16 for the one-tile items, 4 kIrSteps for the runs
+            const int need = m0l - m0f + T.kw + 2;
+            if (!(P->knobs.interp_mfma && need <= (tiles ? 16 : 4 * kIrSteps))) {
                 flush_run();
                 if (run0 < 0) run0 = s;
                 continue;
@@ -783,24 +786,27 @@ static void add_interp_items(Program *P, const std::vector<TfInput> &in)
             flush_valu(s);
             // the tile's window: [mw, mw + 16) for the one-tile items; the run
             // kernel starts it at the device's first tap, which may differ from
-            // the host's m0f by one either way, so runs stage [mw, mw + 18)
+            // the host's m0f by one either way, and reads 4 K steps (16
+            // positions) or, for a wider tile, kIrSteps, so runs stage
+            // [mw, mw + 18) or [mw, mw + 4 kIrSteps + 2)
             const int mw = m0f - 1;
+            const int wdt = need <= 16 ? 18 : 4 * kIrSteps + 2;
             if (tiles) {
                 // one item per tile, every column chunk
                 P->aimt.push_back({box, s, cnt, 0, tb, mw});
                 P->stats.interp_tc_mfma += (long long)cnt * A.E * T.B;
                 continue;
             }
-            if (cur.count && (cur.count == 16 * kIrTiles || mw + 18 - cur.mw0 > kIrPos ||
+            if (cur.count && (cur.count == 16 * kIrTiles || mw + wdt - cur.mw0 > kIrPos ||
                               mw < cur.mw0))
                 flush_run();
             if (!cur.count) {
                 cur.start = s;
                 cur.mw0 = mw;
-                cur.npos = 18;
+                cur.npos = wdt;
             }
             cur.count += cnt;
-            cur.npos = std::max(cur.npos, mw + 18 - cur.mw0);
+            cur.npos = std::max(cur.npos, mw + wdt - cur.mw0);
         }
         flush_run();
         flush_valu(A.ni);
@@ -1094,9 +1100,10 @@ static int build_program(Program *P, const std::vector<TfInput> &in)
                 const NufhtImplHost *h = (const NufhtImplHost *)in[A.tf].plan->impl;
                 double smin = h->w[A.i0] - A.D, smax = h->w[A.i0 + A.ni - 1] - A.D;
                 int mlo = (int)std::ceil((smin - A.a2) / A.delta) - 4;
-                // (up to the last target's first tap + 16: a 16-position
-                // interpolation window may reach that far past the supports)
-                int mhi = (int)std::ceil((smax - A.a2) / A.delta) + std::max(T.kw + 4, 16);
+                // (up to the last target's first tap + 4 kIrSteps + 2: an
+                // interpolation run's window may reach that far past the
+                // supports)
+                int mhi = (int)std::ceil((smax - A.a2) / A.delta) + std::max(T.kw + 4, 4 * kIrSteps + 2);
                 if (mhi - mlo + 1 >= A.Nf) { mlo = 0; mhi = A.Nf - 1; }
                 int4 info = make_int4(A.L, mlo, mhi - mlo, 0);
                 for (long long k = 0; k < ch; ++k) P->fft_chinfo.push_back(info);

@@ -603,7 +603,7 @@ __global__ __launch_bounds__(SpreadSched<SP>::threads, (SP && KW <= 5) ? 4 : 1) 
 
 // Interpolation, target-dense runs, on the matrix cores.  An item is a run
 // of up to kIrTiles consecutive 16-target tiles of one box whose kernel
-// supports each fit a window of 16 grid positions, and whose windows all
+// supports each fit a window of 4 kIrSteps grid positions, and whose windows all
 // lie in [mw0, mw0 + npos), npos <= kIrPos.  Targets are dense against the
 // grid (consecutive tiles mostly share their window: the headline and
 // clustered workloads read each grid position 10 to 13 times as 16-target
@@ -612,7 +612,7 @@ __global__ __launch_bounds__(SpreadSched<SP>::threads, (SP && KW <= 5) ? 4 : 1) 
 // every tile's products read their window from there.  For exponent group
 // g (nsub = 16/tb exponents) and tile u:
 //   D[i][(t, e2)] = sum_m phi(s_i - m Delta) grid[e][t][m],  e = g nsub + e2
-// as 16x16x4 FP64 matrix products (4 K steps of 4 positions, the phi
+// as 16x16x4 FP64 matrix products (up to kIrSteps K steps of 4 positions, the phi
 // operand computed once per tile); the FP64 epilogue applies
 // e^{i(phi0 + w C)} h Delta / psihat(s) and the Hankel weights.  The
 // Hankel combination is folded into a running complex sum per (target,
@@ -679,7 +679,7 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
     const int ia = lane & 15, ka = lane >> 4, jb = lane & 15, tbk = jb % tb, e2b = jb / tb;
     const int ntile = (it.count + 15) >> 4;
     // this wave's tiles u: wave + 4 u; their phi operands and window offsets
-    R phiA[TPW][4];
+    R phiA[TPW][kIrSteps];
     int woff[TPW], nst[TPW];
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
@@ -688,14 +688,14 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
         woff[u] = 0;
         nst[u] = 0;
 #pragma unroll
-        for (int st = 0; st < 4; ++st) phiA[u][st] = (R)0;
+        for (int st = 0; st < kIrSteps; ++st) phiA[u][st] = (R)0;
         if (t < ntile) {
             const int mw = s_m0[b0];                     // the tile's window start
             woff[u] = mw - it.mw0;
             // K steps of 4 positions that hold a kernel tap of some target
             // (the first target's taps start at mw, the last's end at its
             // m0 + kw - 1): the others are all zero and are skipped
-            const int nsteps = min(4, (s_m0[b0 + cnt - 1] + kw - mw + 3) >> 2);
+            const int nsteps = min(kIrSteps, (s_m0[b0 + cnt - 1] + kw - mw + 3) >> 2);
             nst[u] = nsteps;
             // phi(s - m Delta) at tap k = m - m0 of the target (zero outside
             // 0 <= k < kw) from the tap's Chebyshev series in the cell offset
@@ -705,7 +705,7 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
             R tcb[kEsN];
             cheb_basis((R)(2.0 * f - 1.0), tcb);
 #pragma unroll
-            for (int st = 0; st < 4; ++st) {
+            for (int st = 0; st < kIrSteps; ++st) {
                 const int k = mw + 4 * st + ka - mi;
                 const bool on = st < nsteps && ia < cnt && k >= 0 && k < kw;
                 const R v = es_tap(&s_tap[on ? k : 0][0], tcb);
@@ -787,6 +787,11 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
                     const C2 b3 = row[12];
                     dr0 = Mfma<R>::mma(phiA[u][3], (R)b3.x, dr0);
                     di0 = Mfma<R>::mma(phiA[u][3], (R)b3.y, di0);
+                }
+                if (nst[u] > 4) {
+                    const C2 b4 = row[16];
+                    dr0 = Mfma<R>::mma(phiA[u][4], (R)b4.x, dr0);
+                    di0 = Mfma<R>::mma(phiA[u][4], (R)b4.y, di0);
                 }
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
