@@ -220,6 +220,29 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Matrix-core interpolation epilogue: the 16 lanes jb = lane mod 16 of each
+// quarter wave hold rows j = (t, e2) = (jb mod tb, jb / tb), e2 < nsub = 16
+// / tb; sums the nsub parts of each column t into lane jb = t (the other
+// lanes' results are not used).  tb need not be a power of 2.
+__device__ __forceinline__ void fold_exponent_split(double (&out)[4], int tb, int lane)
+{
+    if ((tb & (tb - 1)) == 0) {
+        for (int off = tb; off < 16; off <<= 1)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) out[v] += __shfl_xor(out[v], off, 64);
+        return;
+    }
+    const int nsub = 16 / tb, jb = lane & 15;
+    double o0[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o0[v] = out[v];
+    for (int q = 1; q < nsub; ++q) {
+        const int src = (lane & ~15) | min(jb + q * tb, 15);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[v] += __shfl(o0[v], src, 64);
+    }
+}
+
 constexpr int kVQ = 64;            // sources staged per LDS pass of the VALU spread
 // Exponents per lane of the VALU spread (4 EG accumulators: two columns,
 // complex): channels are split over lanes, so a lane's registers do not grow
@@ -301,6 +324,14 @@ constexpr int kSP = 16;           // accumulator columns: a window of grid posit
 // them) and atomic adds elsewhere (zeroed beforehand).
 struct SpreadRange { int2 piece, wr, ex, pad; };
 
+// Columns an interpolation tile's 16 rows hold: its column chunk (tb, a
+// power of 2) cut to the columns present, so that rows j = (t, e2) = (j mod
+// tbm, j / tbm) carry nsub = 16 / tbm exponents (5 columns: 3 exponents per
+// group rather than 2); rows j >= nsub tbm are idle.
+__host__ __device__ inline int mfma_cols(int tb, int B, int col0)
+{
+    return tb < B - col0 ? tb : B - col0;
+}
 // Products per consumer wave of an item: 2 per exponent group (real and
 // imaginary part), nsub = 16 / tb exponents per group, split over 4 waves.
 __host__ __device__ inline int spread_mfma_kw(int E, int tb)

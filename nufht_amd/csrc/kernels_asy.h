@@ -673,8 +673,9 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
         }
     }
     __syncthreads();
-    int tb = 1;
-    while (tb < T.B && tb < 16) tb <<= 1;
+    // column chunk: all the columns up to 16 (not rounded to a power of 2:
+    // rows j = (t, e2) = (j mod tb, j / tb), e2 < nsub; rows past nsub tb idle)
+    const int tb = T.B < 16 ? T.B : 16;
     const int nsub = 16 / tb, G = (E + nsub - 1) / nsub;
     const int ia = lane & 15, ka = lane >> 4, jb = lane & 15, tbk = jb % tb, e2b = jb / tb;
     const int ntile = (it.count + 15) >> 4;
@@ -736,7 +737,7 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
         for (int r = 0; r < PF; ++r) {
             pf[r].x = (GR)0;
             pf[r].y = (GR)0;
-            if ((pcp[r] >> 16) < 16) {
+            if ((pcp[r] >> 16) < nsub * tb) {
                 const int c = pcp[r] >> 16, p = pcp[r] & 0xffff;
                 const int e = g_ * nsub + c / tb, t = cb_ + c % tb;
                 if (e < E && t < T.B) {
@@ -763,7 +764,7 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
             if (g + 1 < G) load_group(cb, g + 1);
             else if (cb + tb < T.B) load_group(cb + tb, 0);
             const int eb = g * nsub + e2b;
-            const bool eok = (cb + tbk < T.B) && eb < E;
+            const bool eok = (cb + tbk < T.B) && e2b < nsub && eb < E;
             const bool odd = eb & 1;
             const int ec = eok ? eb : 0;
 #pragma unroll
@@ -817,9 +818,7 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
                 const int i = 16 * t + Mfma<R>::row(lane, v);
                 out[v] = t < ntile ? s_b0[i] * (s_fr[i] * Sr[u][v] - s_fi[i] * Si[u][v]) : 0.0;
             }
-            for (int off = tb; off < 16; off <<= 1)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) out[v] += __shfl_xor(out[v], off, 64);
+            fold_exponent_split(out, tb, lane);
             // (s_out[wave] is this wave's own: wave-level ordering suffices)
             wave_lds_sync();
             if (e2b == 0) {
@@ -871,7 +870,7 @@ __global__ __launch_bounds__(64) void k_asy_interp_tile(const TfDesc *__restrict
     AsyBoxD bx = boxes[it.box];
     const TfDesc &T = tfs[bx.tf];
     const int lane = threadIdx.x;
-    const int tb = it.tb, nsub = 16 / tb;
+    const int tb = mfma_cols(it.tb, T.B, it.col0), nsub = 16 / tb;
     const int E = bx.E, kw = T.kw;
     const int mw0 = it.pad;
     if (lane < 16) {
@@ -933,7 +932,7 @@ __global__ __launch_bounds__(64) void k_asy_interp_tile(const TfDesc *__restrict
     double Sr[4] = {0.0, 0.0, 0.0, 0.0}, Si[4] = {0.0, 0.0, 0.0, 0.0};
     for (int g = 0; g < G; ++g) {
         const int eb = g * nsub + e2b;
-        const bool eok = colok && eb < E;
+        const bool eok = colok && e2b < nsub && eb < E;
         // the step's 4 grid values: loads at valid addresses, issued together
         const C2 *row = grid + bx.goff + (long long)((eok ? eb : 0) * T.B + (colok ? tcol : 0)) * bx.Nf;
         C2 b[4];
@@ -981,9 +980,7 @@ __global__ __launch_bounds__(64) void k_asy_interp_tile(const TfDesc *__restrict
         out[v] = s_b0[i] * (s_fr[i] * Sr[v] - s_fi[i] * Si[v]);
     }
     // fold the exponent sub-split (lanes j and j + tb * e2 share a column)
-    for (int off = tb; off < 16; off <<= 1)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) out[v] += __shfl_xor(out[v], off, 64);
+    fold_exponent_split(out, tb, lane);
     // through LDS, so that each atomic instruction adds 16 consecutive rows
     // of 4 columns (4 whole 128-byte lines of g) instead of 4 rows of 16
     if (e2b == 0) {
