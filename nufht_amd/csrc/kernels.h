@@ -29,41 +29,59 @@ __global__ void k_fill0(double *__restrict__ p, long long n)
 constexpr int kDirTJ = 256;
 constexpr int kDirBM = 16;
 
-// J_nu on [0, jtab_n) as piecewise Chebyshev series of degree kJDeg on unit
-// intervals; half-integer orders tabulate J_nu(x) / sqrt(x), which is entire.
+// J_nu on [0, jtab_n): on unit interval iv a polynomial of degree kJDeg in
+// u = x - (iv + 1/2), |u| <= 1/2, held as its monomial coefficients (Horner:
+// one FMA per degree, where the Chebyshev form's Clenshaw recurrence takes
+// two operations).  The polynomial is the interval's Chebyshev interpolant at
+// kJDeg + 1 nodes, its basis changed exactly.  J_nu (integer nu) and
+// J_nu(x) / sqrt(x) (half-integer nu, tabulated so: it is entire) are
+// band-limited to frequencies |s| <= 1, so the Chebyshev coefficients on a
+// unit interval fall as 2 (1/4)^n / n! and the monomial form is well
+// conditioned (its values agree with the Chebyshev form's to a few ulps).
+// Degree 15 keeps relative accuracy where J_nu is tiny (x < nu: J_10 on
+// [0, 1] is below 3e-10), which degree 11 would lose.
 constexpr int kJDeg = 15;
 
-__device__ __forceinline__ double jtab_eval(const double *__restrict__ tab, double x)
+__device__ __forceinline__ double jtab_horner(const double (&c)[kJDeg + 1], double u)
 {
-    int iv = (int)x;
-    double t = 2.0 * (x - (double)iv) - 1.0;
-    const double *c = tab + (long long)iv * (kJDeg + 1);
-    double b1 = 0.0, b2 = 0.0;
+    double p = c[kJDeg];
 #pragma unroll
-    for (int n = kJDeg; n >= 1; --n) {
-        double b0 = fma(2.0 * t, b1, c[n] - b2);
-        b2 = b1;
-        b1 = b0;
-    }
-    return fma(t, b1, c[0] - b2);
+    for (int n = kJDeg - 1; n >= 0; --n) p = fma(p, u, c[n]);
+    return p;
 }
 
 // Table build: one block of kJDeg + 1 threads per unit interval; values of
-// the exact routine at Chebyshev nodes, then the discrete cosine transform.
+// the exact routine at Chebyshev nodes, the discrete cosine transform
+// (Chebyshev coefficients in t = 2u), then the exact change of basis to
+// monomials in u: T_k(2u) = sum_n M[k][n] u^n.
 __global__ void k_jtab_build(int order, int half, int nint, double *__restrict__ tab)
 {
-    __shared__ double f[kJDeg + 1];
-    int iv = blockIdx.x, k = threadIdx.x;
     constexpr int N = kJDeg + 1;
+    __shared__ double f[N], ch[N], M[N][N];
+    int iv = blockIdx.x, k = threadIdx.x;
     double tk = cos(kPi * ((double)k + 0.5) / N);
     double x = (double)iv + 0.5 * (tk + 1.0);
     double v = half ? jn_half(order, x, 0.0) : jn_int(order, x, 0.0);
     if (half) v = x > 0.0 ? v / sqrt(x) : 0.0;
     f[k] = v;
+    if (k == 0) {
+        // T_0 = 1, T_1 = 2u, T_{j+1} = 4u T_j - T_{j-1} (integers, exact)
+        for (int j = 0; j < N; ++j)
+            for (int n = 0; n < N; ++n) M[j][n] = 0.0;
+        M[0][0] = 1.0;
+        M[1][1] = 2.0;
+        for (int j = 1; j + 1 < N; ++j)
+            for (int n = 0; n < N; ++n)
+                M[j + 1][n] = (n > 0 ? 4.0 * M[j][n - 1] : 0.0) - M[j - 1][n];
+    }
     __syncthreads();
     double acc = 0.0;
     for (int q = 0; q < N; ++q) acc += f[q] * cos(kPi * (double)k * ((double)q + 0.5) / N);
-    tab[(long long)iv * N + k] = acc * (k == 0 ? 1.0 : 2.0) / N;
+    ch[k] = acc * (k == 0 ? 1.0 : 2.0) / N;
+    __syncthreads();
+    double a = 0.0;
+    for (int j = N - 1; j >= k; --j) a += ch[j] * M[j][k];
+    tab[(long long)iv * N + k] = a;
 }
 
 // NB: the item's columns, rounded up to 4, 8 or 16 (one kernel per class).
@@ -135,20 +153,12 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
                     if (!(x >= 0.0 && x < xmax)) return besselj_wr(T.order, T.half, w, s_r[j]);
                     const int iv = (int)x;
                     load_iv(iv);
-                    const double tt = 2.0 * (x - (double)iv) - 1.0;
-                    double b1 = 0.0, b2 = 0.0;
-#pragma unroll
-                    for (int n = kJDeg; n >= 1; --n) {
-                        const double b0 = fma(2.0 * tt, b1, cc[n] - b2);
-                        b2 = b1;
-                        b1 = b0;
-                    }
-                    const double J = fma(tt, b1, cc[0] - b2);
+                    const double J = jtab_horner(cc, x - ((double)iv + 0.5));
                     return T.half ? J * sqrt(x) : J;
                 };
                 int j = ja;
                 // 16 columns: two sources at a time; in one interval (the
-                // common case) their Clenshaw recurrences run interleaved
+                // common case) their Horner chains run interleaved
                 // (for fewer columns the second chain's registers cost more
                 // occupancy than the interleaving gains)
                 for (; NB == 16 && j + 1 < jb; j += 2) {
@@ -157,20 +167,9 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
                     const int iv0 = (int)x0;
                     if (x0 >= 0.0 && x1 < xmax && iv0 == (int)x1) {
                         load_iv(iv0);
-                        const double t0 = 2.0 * (x0 - (double)iv0) - 1.0;
-                        const double t1 = 2.0 * (x1 - (double)iv0) - 1.0;
-                        double a1 = 0.0, a2 = 0.0, c1 = 0.0, c2 = 0.0;
-#pragma unroll
-                        for (int n = kJDeg; n >= 1; --n) {
-                            const double a0 = fma(2.0 * t0, a1, cc[n] - a2);
-                            const double c0 = fma(2.0 * t1, c1, cc[n] - c2);
-                            a2 = a1;
-                            a1 = a0;
-                            c2 = c1;
-                            c1 = c0;
-                        }
-                        J0 = fma(t0, a1, cc[0] - a2);
-                        J1 = fma(t1, c1, cc[0] - c2);
+                        const double mid = (double)iv0 + 0.5;
+                        J0 = jtab_horner(cc, x0 - mid);
+                        J1 = jtab_horner(cc, x1 - mid);
                         if (T.half) {
                             J0 *= sqrt(x0);
                             J1 *= sqrt(x1);
