@@ -635,7 +635,7 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
     typedef typename Mfma<R>::V V;
     constexpr int TPW = kIrTiles / 4;
     __shared__ C2 s_g[16 * kIrStride];           // staged window: [channel][position]
-    __shared__ double s_s[kIrTargets], s_fr[kIrTargets], s_fi[kIrTargets], s_b0[kIrTargets];
+    __shared__ double s_s[kIrTargets], s_fr[kIrTargets], s_fi[kIrTargets];
     __shared__ int s_m0[kIrTargets];
     __shared__ double s_P[kIrTargets][EMAX];
     __shared__ double s_out[4][16][17];           // per wave: result tile, transposed
@@ -650,8 +650,20 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
         s_tap[k][q - k * kEsN] = ft[T.ft_off + kNFT + q];
     }
     // per-target factors, one thread per target (rows past the run's end
-    // repeat its last target with zero weight)
-    for (int ti = tid; ti < kIrTargets; ti += 256) {
+    // repeat its last target with zero weight); the Hankel weights' powers
+    // on the block's other half
+    static_assert(2 * kIrTargets == 256, "one thread per target in each half");
+    if (tid >= kIrTargets) {
+        const int ti = tid - kIrTargets;
+        const double w = T.w[bx.i0 + it.start + min(ti, it.count - 1)];
+        const double rw = 1.0 / (bx.rho * w);
+        double pw = 1.0;
+        for (int e = 0; e < EMAX; ++e) {
+            s_P[ti][e] = e < E ? ((((e >> 1) & 1) ? -T.a[e] : T.a[e]) * pw) : 0.0;
+            pw *= rw;
+        }
+    } else {
+        const int ti = tid;
         const double w = T.w[bx.i0 + it.start + min(ti, it.count - 1)];
         const double s = w - bx.D;
         s_s[ti] = s;
@@ -662,15 +674,10 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
         const int q = (int)lround(2.0 * nu + 1.0) & 7;
         const double c0 = cos_qpi4(q), s0 = -cos_qpi4((q + 6) & 7);
         const double amp = bx.h * bx.delta / (bx.a1 * ft_eval(ft + T.ft_off, s * bx.a1, T.kmax));
-        s_fr[ti] = (cp * c0 - sp * s0) * amp;
-        s_fi[ti] = (sp * c0 + cp * s0) * amp;
-        s_b0[ti] = ti < it.count ? T.sign * sqrt(2.0 / kPi) / sqrt(w) : 0.0;
-        const double rw = 1.0 / (bx.rho * w);
-        double pw = 1.0;
-        for (int e = 0; e < EMAX; ++e) {
-            s_P[ti][e] = e < E ? ((((e >> 1) & 1) ? -T.a[e] : T.a[e]) * pw) : 0.0;
-            pw *= rw;
-        }
+        // with the Hankel amplitude b0 = +-sqrt(2 / (pi w)) folded in
+        const double b0 = ti < it.count ? T.sign * sqrt(2.0 / kPi) / sqrt(w) : 0.0;
+        s_fr[ti] = (cp * c0 - sp * s0) * amp * b0;
+        s_fi[ti] = (sp * c0 + cp * s0) * amp * b0;
     }
     __syncthreads();
     // column chunk: all the columns up to 16 (not rounded to a power of 2:
@@ -816,7 +823,7 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = 16 * t + Mfma<R>::row(lane, v);
-                out[v] = t < ntile ? s_b0[i] * (s_fr[i] * Sr[u][v] - s_fi[i] * Si[u][v]) : 0.0;
+                out[v] = t < ntile ? s_fr[i] * Sr[u][v] - s_fi[i] * Si[u][v] : 0.0;
             }
             fold_exponent_split(out, tb, lane);
             // (s_out[wave] is this wave's own: wave-level ordering suffices)
