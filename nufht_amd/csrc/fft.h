@@ -34,6 +34,30 @@ template <typename R> struct CMul {
     static __device__ __forceinline__ C2 mul_i(C2 a) { C2 z; z.x = -a.y; z.y = a.x; return z; }
 };
 
+// Twiddles w^r, r = 1 .. RAD - 1, of a butterfly into x[1 .. RAD - 1]
+// (the table holds w^m at index m).  FP64: one table load, the powers by
+// complex products (w^(2s) = (w^s)^2, w^(2s+1) = w^(2s) w; at most four
+// roundings deep, about 1e-15), which costs less than RAD - 1 loads through
+// the vector cache.  FP32 keeps one load per power: its products' rounding
+// would count against the mixed mode's error budget.
+template <typename R, int RAD>
+__device__ __forceinline__ void fft_twiddle(typename Cplx<R>::T (&x)[RAD],
+                                            const typename Cplx<R>::T *__restrict__ W, int m)
+{
+    typedef typename Cplx<R>::T C2;
+    if constexpr (sizeof(R) == 8) {
+        C2 w[RAD];
+        w[1] = W[m];
+#pragma unroll
+        for (int r = 2; r < RAD; ++r) w[r] = CMul<R>::mul(w[r >> 1], w[(r + 1) >> 1]);
+#pragma unroll
+        for (int r = 1; r < RAD; ++r) x[r] = CMul<R>::mul(x[r], w[r]);
+    } else {
+#pragma unroll
+        for (int r = 1; r < RAD; ++r) x[r] = CMul<R>::mul(x[r], W[r * m]);
+    }
+}
+
 // small DFTs with the backward sign
 template <typename R, int RAD> struct Dft;
 template <typename R> struct Dft<R, 2> {
@@ -166,10 +190,7 @@ __device__ __forceinline__ void fft_stage(typename Cplx<R>::T *s, int N, int M, 
             int c = b & (M - 1), j = b >> lgM, k = j & (Ns - 1);
 #pragma unroll
             for (int r = 0; r < RAD; ++r) x[q][r] = s[fft_lds_idx(j + r * span, c, ls, lh)];
-            if (Ns > 1) {
-#pragma unroll
-                for (int r = 1; r < RAD; ++r) x[q][r] = CMul<R>::mul(x[q][r], W[r * k * wstep]);
-            }
+            if (Ns > 1) fft_twiddle<R, RAD>(x[q], W, k * wstep);
             Dft<R, RAD>::run(x[q]);
         }
     }
@@ -252,8 +273,7 @@ __device__ __forceinline__ void fft_last_stage(const typename Cplx<R>::T *s, con
             C2 x[RAD];
 #pragma unroll
             for (int r = 0; r < RAD; ++r) x[r] = s[fft_lds_idx(j + r * Ns, c, P.lds_stride, P.lds_shift)];
-#pragma unroll
-            for (int r = 1; r < RAD; ++r) x[r] = CMul<R>::mul(x[r], W[r * j]);
+            fft_twiddle<R, RAD>(x, W, j);
             Dft<R, RAD>::run(x);
 #pragma unroll
             for (int r = 0; r < RAD; ++r) st(c, j + r * Ns, x[r]);
