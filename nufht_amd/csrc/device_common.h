@@ -211,6 +211,25 @@ __device__ __forceinline__ float es_kernel_f(float u, float beta)
     return t > 0.0f ? __expf(beta * (__fsqrt_rn(t) - 1.0f)) : 0.0f;
 }
 
+// A load through a global-address-space pointer.  Pointers read from a
+// descriptor in memory are generic, and generic loads (flat) count against
+// the LDS / scalar wait counter as well, so that a later wait for an LDS
+// read or a scalar load also waits for them; global loads count only
+// against the vector-memory counter, and loads issued ahead stay in flight.
+template <typename T>
+__device__ __forceinline__ T ld_global(const T *p)
+{
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
+// An FP64 atomic add into global memory through a global-address-space
+// pointer (a global atomic, not a flat one: see ld_global).
+__device__ __forceinline__ void atomic_add_global(double *p, double v)
+{
+    __hip_atomic_fetch_add((__attribute__((address_space(1))) double *)p, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Orders one wave's LDS writes before its later LDS reads (and reads before
 // later writes) where only that wave touches the data: no block barrier.
 __device__ __forceinline__ void wave_lds_sync()

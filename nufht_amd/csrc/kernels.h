@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
     for (int t = 0; t < NB; ++t) acc[t] = 0.0;
     const double *tab = jtab + T.jtab_off;
     double xmax = (double)T.jtab_n;
-    const double w = active ? T.w[it.i0 + row] : 0.0;
+    const double w = active ? ld_global(&T.w[it.i0 + row]) : 0.0;
     int civ = -1;                   // unit interval of the coefficients held in cc
     double cc[kJDeg + 1];
 #pragma unroll
@@ -129,8 +129,8 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
                 double cv[NB];
 #pragma unroll
                 for (int t = 0; t < NB; ++t)
-                    cv[t] = (ok && t < nb) ? T.cs[(long long)(it.col0 + t) * T.ld_cs + j0 + j] : 0.0;
-                s_r[j] = ok ? T.r[j0 + j] : 0.0;
+                    cv[t] = (ok && t < nb) ? ld_global(&T.cs[(long long)(it.col0 + t) * T.ld_cs + j0 + j]) : 0.0;
+                s_r[j] = ok ? ld_global(&T.r[j0 + j]) : 0.0;
 #pragma unroll
                 for (int t = 0; t < NB; t += 2) *(double2 *)&s_c[j * LDC + t] = make_double2(cv[t], cv[t + 1]);
             }
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
     }
     if (active && lane == 0) {
         for (int t = 0; t < nb; ++t)
-            atomicAdd(&T.gs[(long long)(it.col0 + t) * T.ld_gs + it.i0 + row], T.sign * acc[t]);
+            atomic_add_global(&T.gs[(long long)(it.col0 + t) * T.ld_gs + it.i0 + row], T.sign * acc[t]);
     }
 }
 
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_loc_proj(const TfDesc *__restrict__ tfs
         if (tid < kLocSub) {
             LR *row = sT + (size_t)tid * LT;
             if (tid < n) {
-                double x = T.r[bx.j0 + it.start + s0 + tid] / bx.r_end;
+                double x = ld_global(&T.r[bx.j0 + it.start + s0 + tid]) / bx.r_end;
                 x = fmin(1.0, fmax(-1.0, x));
                 // a_l = T_{2l+par}(x) by a_{l+1} = 2 T_2 a_l - a_{l-1} (from
                 // T_m T_n = (T_{m+n} + T_{|m-n|}) / 2), a_{-1} = T_{par-2} =
@@ -292,8 +292,8 @@ __global__ __launch_bounds__(256) void k_loc_proj(const TfDesc *__restrict__ tfs
             double c = 0.0;
             if (q < n && t < nb) {
                 int j = bx.j0 + it.start + s0 + q;
-                c = T.cs[(long long)(it.col0 + t) * T.ld_cs + j];
-                if (T.half) c *= sqrt(T.r[j]);
+                c = ld_global(&T.cs[(long long)(it.col0 + t) * T.ld_cs + j]);
+                if (T.half) c *= sqrt(ld_global(&T.r[j]));
             }
             sC[q * LC + t] = (LR)c;
         }
@@ -429,10 +429,10 @@ __global__ __launch_bounds__(256) void k_loc_eval1(const TfDesc *__restrict__ tf
                 if (nt > 3) a3 = fma(bl, (LR)cr[3], a3);
             }
             double *g = T.gs + (long long)t0 * T.ld_gs + i;
-            atomicAdd(g, sc0 * (double)a0);
-            if (nt > 1) atomicAdd(g + T.ld_gs, sc0 * (double)a1);
-            if (nt > 2) atomicAdd(g + 2 * (long long)T.ld_gs, sc0 * (double)a2);
-            if (nt > 3) atomicAdd(g + 3 * (long long)T.ld_gs, sc0 * (double)a3);
+            atomic_add_global(g, sc0 * (double)a0);
+            if (nt > 1) atomic_add_global(g + T.ld_gs, sc0 * (double)a1);
+            if (nt > 2) atomic_add_global(g + 2 * (long long)T.ld_gs, sc0 * (double)a2);
+            if (nt > 3) atomic_add_global(g + 3 * (long long)T.ld_gs, sc0 * (double)a3);
         }
         return;
     }
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(256) void k_loc_eval1(const TfDesc *__restrict__ tf
         double *g = T.gs + (long long)t0 * T.ld_gs + i;
 #pragma unroll
         for (int t = 0; t < 8; ++t)
-            if (t < nt) atomicAdd(g + (long long)t * T.ld_gs, sc * (double)a[t]);
+            if (t < nt) atomic_add_global(g + (long long)t * T.ld_gs, sc * (double)a[t]);
     }
 }
 
@@ -584,7 +584,7 @@ __global__ __launch_bounds__(256) void k_loc_eval(const TfDesc *__restrict__ tfs
     double *g = T.gs + (long long)it.col0 * T.ld_gs + i;
 #pragma unroll
     for (int t = 0; t < BMAX; ++t)
-        if (t < nb) atomicAdd(g + (long long)t * T.ld_gs, sc * (double)acc[t]);
+        if (t < nb) atomic_add_global(g + (long long)t * T.ld_gs, sc * (double)acc[t]);
 }
 
 

@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256, 4) void k_asy_spread(const TfDesc *__restrict_
     for (int c0 = jr.x; c0 < jr.y; c0 += kVQ) {
         const int cn = min(kVQ, jr.y - c0);
         if (tid < cn) {
-            const double rj = T.r[c0 + tid];
+            const double rj = ld_global(&T.r[c0 + tid]);
             const double x = rj - bx.C;
             s_x[tid] = x;
             s_pf[tid] = (int)ceil((x - bx.a1) / bx.h);
@@ -118,8 +118,8 @@ __global__ __launch_bounds__(256, 4) void k_asy_spread(const TfDesc *__restrict_
             const int pr = q / cn, jq = q - pr * cn;
             const int ta = it.col0 + 2 * pr, tb2 = ta + 1;
             C2 v;
-            v.x = ta < T.B && 2 * pr < tb ? (R)T.cs[(long long)ta * T.ld_cs + c0 + jq] : (R)0;
-            v.y = tb2 < T.B && 2 * pr + 1 < tb ? (R)T.cs[(long long)tb2 * T.ld_cs + c0 + jq] : (R)0;
+            v.x = ta < T.B && 2 * pr < tb ? (R)ld_global(&T.cs[(long long)ta * T.ld_cs + c0 + jq]) : (R)0;
+            v.y = tb2 < T.B && 2 * pr + 1 < tb ? (R)ld_global(&T.cs[(long long)tb2 * T.ld_cs + c0 + jq]) : (R)0;
             s_c[pr][jq] = v;
         }
         __syncthreads();
@@ -278,12 +278,12 @@ __device__ __forceinline__ void spread_load(const TfDesc &T, const AsyItem &it, 
                                             int ptid, SpreadIn<CV> &in)
 {
     const int q = ptid & 63;
-    in.r = T.r[c0 + min(q, cn - 1)];
+    in.r = ld_global(&T.r[c0 + min(q, cn - 1)]);
     if constexpr (CV) {
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
             const int j = (ptid - 128 + 128 * m) >> 6, tc = it.col0 + j % it.tb;
-            in.cv[m] = (q < cn && tc < T.B) ? T.cs[(long long)tc * T.ld_cs + c0 + q] : 0.0;
+            in.cv[m] = (q < cn && tc < T.B) ? ld_global(&T.cs[(long long)tc * T.ld_cs + c0 + q]) : 0.0;
         }
     }
 }
@@ -655,7 +655,7 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
     static_assert(2 * kIrTargets == 256, "one thread per target in each half");
     if (tid >= kIrTargets) {
         const int ti = tid - kIrTargets;
-        const double w = T.w[bx.i0 + it.start + min(ti, it.count - 1)];
+        const double w = ld_global(&T.w[bx.i0 + it.start + min(ti, it.count - 1)]);
         const double rw = 1.0 / (bx.rho * w);
         double pw = 1.0;
         for (int e = 0; e < EMAX; ++e) {
@@ -664,7 +664,7 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
         }
     } else {
         const int ti = tid;
-        const double w = T.w[bx.i0 + it.start + min(ti, it.count - 1)];
+        const double w = ld_global(&T.w[bx.i0 + it.start + min(ti, it.count - 1)]);
         const double s = w - bx.D;
         s_s[ti] = s;
         s_m0[ti] = (int)ceil((s - bx.a2) / bx.delta);
@@ -837,7 +837,7 @@ __global__ __launch_bounds__(256) void k_asy_interp_mfma(const TfDesc *__restric
                 for (int idx = lane; idx < 16 * tb; idx += 64) {
                     const int c = idx >> 4, i = idx & 15;
                     if (16 * t + i < it.count && cb + c < T.B)
-                        atomicAdd(&T.gs[(long long)(cb + c) * T.ld_gs + bx.i0 + it.start + 16 * t + i],
+                        atomic_add_global(&T.gs[(long long)(cb + c) * T.ld_gs + bx.i0 + it.start + 16 * t + i],
                                   s_out[wave][i][c]);
                 }
             }
@@ -998,7 +998,7 @@ __global__ __launch_bounds__(64) void k_asy_interp_tile(const TfDesc *__restrict
     for (int idx = lane; idx < 16 * tb; idx += 64) {
         const int c = idx >> 4, i = idx & 15;
         if (i < it.count && cb + c < T.B)
-            atomicAdd(&T.gs[(long long)(cb + c) * T.ld_gs + bx.i0 + it.start + i], s_out[i][c]);
+            atomic_add_global(&T.gs[(long long)(cb + c) * T.ld_gs + bx.i0 + it.start + i], s_out[i][c]);
     }
     __syncthreads();
     }
@@ -1157,7 +1157,7 @@ __global__ __launch_bounds__(256) void k_asy_interp(const TfDesc *__restrict__ t
         }
     }
     if (active)
-        atomicAdd(&T.gs[(long long)t * T.ld_gs + bx.i0 + it.start + ti], fr * Sr - fi * Si);
+        atomic_add_global(&T.gs[(long long)t * T.ld_gs + bx.i0 + it.start + ti], fr * Sr - fi * Si);
 }
 
 }  // namespace nufht_dev
