@@ -57,8 +57,8 @@ def stage_work(st: dict) -> dict:
     plan statistics of the program that ran it.
 
     A channel is one complex series (exponent e, column t); a box has
-    C = 2 (K_asy + 1) B of them.  The per-unit figures are the survey's
-    cost model: 2 w flops per (source, channel) spread, 4 w flops per
+    C = 2 (K_asy + 1) B of them.  The per-unit figures are the algorithm's
+    operation counts: 2 w flops per (source, channel) spread, 4 w flops per
     (target, channel) interpolated (w = kernel width), 2 B flops per direct
     (target, source) pair with the Bessel evaluation not counted, 2 (K_loc +
     1) B flops per local-expansion source (projection) and target
@@ -80,7 +80,7 @@ def stage_work(st: dict) -> dict:
 
 
 def path_model(stats: list, t_measured_ms: float) -> dict:
-    """Whole-path roofline of one step: the survey's algorithmic work of every
+    """Whole-path roofline of one step: the algorithmic work of every
     application the step runs (FP64 and FP32 flops, HBM bytes; the plan
     statistics' model_* totals), its floor t_roof = max(F64 / 78.6 TFLOP/s +
     F32 / 157.3 TFLOP/s, bytes / 8 TB/s) and achieved = t_roof / t_measured."""
