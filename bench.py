@@ -521,7 +521,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build "
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): it "
               "measures one GPU's share of the transform; sharding is a design note.")
         sys.exit(2)
     import torch

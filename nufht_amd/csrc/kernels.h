@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void k_dir(const TfDesc *__restrict__ tfs,
 // Local expansion, projection half: coef[l][t] += sum_j B_l(x_j) c_j,t with
 // x_j = r_j / r_end.  Integer orders follow the reference's Wimp expansion
 // (src/expansions.c:242-273): even nu -> eps_l T_2l(x), odd nu -> 2 T_2l+1(x).
-// Half-integer orders (Chebyshev interpolation of j_n, this build) use
+// Half-integer orders (Chebyshev interpolation of j_n, not the reference's form) use
 // T_2l+p(x), p = n mod 2, with weights c_j sqrt(r_j).
 //
 // The contraction is a (K+1) x count x 16 matrix product per item, run on

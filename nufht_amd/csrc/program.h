@@ -81,7 +81,7 @@ struct Knobs {
 struct FftClass {
     int nf = 0;
     long long batch = 0, off = 0;
-    bool own = false;          // this build's FFT kernels (else rocFFT, out of place)
+    bool own = false;          // the two-pass FFT kernels of fft.h (else rocFFT, out of place)
     bool small = false;        // one pass (p1 only): lengths up to one LDS tile
     FftPass p1, p2;
     size_t lds1 = 0, lds2 = 0;

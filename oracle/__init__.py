@@ -4,7 +4,7 @@ A CPU restatement of the reference's ``nufht_batch`` path (Ansal06/NUFHT
 ``src/fast_hankel_nufht.c``, ``src/expansions.c``, ``src/bounds.c``) in plain C
 (``oracle/*.c``), plus an FP64 direct-sum reference.  Only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
-this package, and only as the checker or as the timed CPU baseline.
+this package, and only to check results or as the timed CPU baseline.
 
 ``load_ref_planner()`` returns the reference's own planner compiled from its
 sources into ``oracle/_ref`` (see ``oracle/Makefile``), or ``None``.

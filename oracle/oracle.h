@@ -4,8 +4,8 @@
  * A plain-C, CPU-only restatement of the reference's nufht_batch path
  * (Ansal06/NUFHT src/fast_hankel_nufht.c, src/expansions.c, src/bounds.c)
  * plus an FP64 direct-sum reference.  Only tests/, __graft_entry__.smoke()
- * and bench.py's cpu_baseline leg may load it, and only as the checker or
- * the timed CPU baseline.  Nothing in nufht_amd/ includes or links it.
+ * and bench.py's cpu_baseline leg may load it, and only to check results or
+ * as the timed CPU baseline.  Nothing in nufht_amd/ includes or links it.
  */
 #ifndef NUFHT_ORACLE_H
 #define NUFHT_ORACLE_H

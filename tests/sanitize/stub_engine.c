@@ -30,6 +30,6 @@ int nufht_engine_batch_host(const NufhtPlan *plan, const double *cs, int ld_cs, 
                             int ld_gs, int batch, NufhtScratch *scratch)
 {
     (void)plan; (void)cs; (void)ld_cs; (void)gs; (void)ld_gs; (void)batch; (void)scratch;
-    nufht_set_error("no device in this build");
+    nufht_set_error("no device in the sanitizer build");
     return -510;
 }
